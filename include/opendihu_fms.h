@@ -1,0 +1,173 @@
+/*
+ * C ABI of the MI355X FastMonodomainSolver fibre library (libfms_<model>.so).
+ *
+ * Two layers:
+ *
+ * 1. Reference plugin boundary. OpenDiHu's FastMonodomainSolver with
+ *    CellML.optimizationType = "gpu" and CellML.libraryFilename set dlopens a
+ *    prebuilt library and binds exactly two symbols
+ *    (core/src/specialized_solver/fast_monodomain_solver/fast_monodomain_solver_gpu.tpp:184-205):
+ *      initializeArrays   (fast_monodomain_solver_gpu.tpp:356-365, called once at :1003-1007)
+ *      computeMonodomain  (fast_monodomain_solver_gpu.tpp:478-481, called per advanceTimeSpan at :1078-1081)
+ *    Their sizes are compile-time constants of the reference's generated source
+ *    (fast_monodomain_solver_gpu.tpp:293-312); here they come from
+ *    fms_reference_set_dims() or the OPENDIHU_FMS_DIMS environment variable.
+ *
+ * 2. Native API (fms_*): runtime sizes, fibres of uneven length, FP64 element
+ *    lengths and per-fibre parameters, firing-table or current-pulse
+ *    stimulation, FP64 or mixed precision, split or fused kernels. The handle
+ *    owns all device state; calls on one handle are not re-entrant.
+ *
+ * All functions return 0 on success and a negative FMS_ERR_* code on failure;
+ * fms_last_error() gives the message. Nothing throws across this boundary.
+ * The reference-compatible void functions report failure through
+ * fms_last_error() only (the reference binding has no return value).
+ */
+#ifndef OPENDIHU_FMS_H
+#define OPENDIHU_FMS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMS_OK 0
+#define FMS_ERR_ARGUMENT (-1)
+#define FMS_ERR_HIP (-2)
+#define FMS_ERR_STATE (-3)
+#define FMS_ERR_UNSUPPORTED (-4)
+
+/* diffusion time stepping (DiffusionTimeSteppingScheme of FastMonodomainSolverBase) */
+#define FMS_IMPLICIT_EULER 0
+#define FMS_CRANK_NICOLSON 1
+/* state precision: FP64 everywhere, or FP32 gates with FP64 Vm and FP64 Thomas */
+#define FMS_FP64 0
+#define FMS_MIXED 1
+/* exp/log: CellML.approximateExponentialFunction = True / False */
+#define FMS_EXP_APPROXIMATE 0
+#define FMS_EXP_LIBM 1
+/* tridiagonal solver: reference recurrence (bit-exact) / partitioned + PCR (fast) */
+#define FMS_THOMAS_SERIAL 0
+#define FMS_THOMAS_PARTITIONED 1
+/* kernel schedule: separate reaction/diffusion kernels, or one fused multi-step kernel */
+#define FMS_KERNELS_SPLIT 0
+#define FMS_KERNELS_FUSED 1
+/* stimulation */
+#define FMS_STIM_NONE 0
+#define FMS_STIM_FIRING_TABLE 1
+#define FMS_STIM_CURRENT 2
+
+typedef struct FmsSolver FmsSolver;
+
+typedef struct {
+  int n_fibers;
+  const int *n_nodes;              /* [n_fibers], each >= 2 */
+  const double *element_lengths;   /* [sum(n_nodes-1)], fibre after fibre */
+  const double *prefactor;         /* [n_fibers], sigma/(Am*Cm) (FiniteElementMethod.prefactor) */
+  const double *parameters;        /* [n_fibers*n_params] or NULL for the model's defaults */
+  const double *initial_states;    /* [n_states] or NULL for the model's defaults */
+  int scheme;                      /* FMS_IMPLICIT_EULER / FMS_CRANK_NICOLSON */
+  int precision;                   /* FMS_FP64 / FMS_MIXED */
+  int exp_mode;                    /* FMS_EXP_APPROXIMATE / FMS_EXP_LIBM */
+  int thomas;                      /* FMS_THOMAS_SERIAL / FMS_THOMAS_PARTITIONED */
+  int kernels;                     /* FMS_KERNELS_SPLIT / FMS_KERNELS_FUSED */
+  double value_for_stimulated_point; /* valueForStimulatedPoint (default 20) */
+} FmsConfig;
+
+typedef struct {
+  int mode;                        /* FMS_STIM_* */
+  const int *stim_begin;           /* [n_fibers] first stimulated node (fibre-local) */
+  const int *stim_end;             /* [n_fibers] one past the last stimulated node */
+  /* FMS_STIM_FIRING_TABLE: firingTimesFile table and setSpecificStates* settings */
+  int firing_rows, firing_cols;
+  const unsigned char *firing_events;   /* [rows*cols], row = sampling step, col = motor unit */
+  const int *motor_unit;                /* [n_fibers] */
+  const double *call_frequency;         /* [n_fibers] setSpecificStatesCallFrequency */
+  const double *repeat_after_first_call;/* [n_fibers] setSpecificStatesRepeatAfterFirstCall */
+  const double *call_enable_begin;      /* [n_fibers] setSpecificStatesCallEnableBegin */
+  const double *jitter;                 /* concatenated setSpecificStatesFrequencyJitter lists */
+  const int *jitter_offset;             /* [n_fibers] */
+  const int *jitter_count;              /* [n_fibers] */
+  const double *last_check_time;        /* [n_fibers] lastStimulationCheckTime, or NULL for
+                                           enableBegin - 1e-13 - 1/frequency */
+  int log_capacity;                     /* stimulation-onset log entries kept (0 = no log) */
+  /* FMS_STIM_CURRENT: parameter stim_param := amplitude on [onset, onset+duration) */
+  const double *onset, *duration, *amplitude;  /* [n_fibers] */
+  int stim_param;
+} FmsStimulation;
+
+/* model compiled into this library */
+int fms_model_info(int *n_states, int *n_algebraics, int *n_params, char *name, int name_len);
+int fms_model_defaults(double *states /* [n_states] */, double *params /* [n_params] */);
+
+int fms_create(const FmsConfig *config, FmsSolver **solver);
+int fms_destroy(FmsSolver *solver);
+int fms_set_stimulation(FmsSolver *solver, const FmsStimulation *stim);
+
+/* computeMonodomain semantics: n_split Strang steps from start_time; one
+   diffusion solve with dt1D per step (n1D is accepted and, as in the
+   reference, not looped over). Enqueued on the solver's stream. */
+int fms_advance(FmsSolver *solver, double start_time, double dt_split, int n_split,
+                double dt0D, int n0D, double dt1D, int n1D);
+/* same, with per-step start times and widths (RepeatedCall schedules):
+   steps[k*4 + {0,1,2,3}] = {start_time, dt_split, dt0D, dt1D}; n0D Heun steps per half */
+int fms_advance_schedule(FmsSolver *solver, const double *steps, int n_steps, int n0D);
+int fms_synchronize(FmsSolver *solver);
+
+int fms_get_states(FmsSolver *solver, int state_no, double *out /* [n_total] */);
+int fms_set_states(FmsSolver *solver, int state_no, const double *in /* [n_total] */);
+int fms_get_algebraics(FmsSolver *solver, double *out /* [n_alg_transfer*n_total] */);
+int fms_set_algebraics_for_transfer(FmsSolver *solver, const int *indices, int count);
+int fms_total_nodes(FmsSolver *solver, long long *n_total);
+
+/* spike detector: first split step (1-based, counted from creation) at which
+   Vm reaches threshold from below, and number of upward crossings, per node */
+int fms_enable_spike_detection(FmsSolver *solver, double threshold);
+int fms_get_spikes(FmsSolver *solver, int *first_step, int *count);
+int fms_get_stimulation_log(FmsSolver *solver, int *count, double *times, int *fibers, int capacity);
+
+/* RepeatedCall<FastMonodomainSolver> time-step bookkeeping
+   (core/src/time_stepping_scheme/repeated_call.tpp:45-80,
+   core/src/time_stepping_scheme/00_time_stepping_scheme.cpp:15-70):
+   fills out[k*7 + {0..6}] = {startTime, dtSplit, nSplit, dt0D, n0D, dt1D, n1D}
+   for each advanceTimeSpan call; returns the number of calls (or < 0). */
+int fms_schedule(double rc_end, double rc_width, double strang_end, double strang_width,
+                 double heun_end, double heun_width, double ie_end, double ie_width,
+                 int max_calls, double *out);
+
+int fms_last_error(char *buf, int len);
+
+/* ---- reference plugin boundary ---- */
+typedef struct {
+  int n_fibers_to_compute;
+  int n_instances_per_fiber;
+  int n_parameters_per_instance;
+  int firing_events_rows;
+  int firing_events_cols;
+  int frequency_jitter_cols;
+  int n_algebraics_for_transfer;
+  int n_states_for_transfer;
+  int scheme;                      /* FMS_IMPLICIT_EULER / FMS_CRANK_NICOLSON */
+  int precision;                   /* useSinglePrecision -> FMS_MIXED */
+  int exp_mode;
+} FmsReferenceDims;
+
+int fms_reference_set_dims(const FmsReferenceDims *dims);
+
+void initializeArrays(const double *statesOneInstance, const int *algebraicsForTransferIndices,
+                      const int *statesForTransferIndices, const char *firingEvents,
+                      const double *setSpecificStatesFrequencyJitter, const int *motorUnitNo,
+                      const int *fiberStimulationPointIndex, const double *lastStimulationCheckTime,
+                      const double *setSpecificStatesCallFrequency,
+                      const double *setSpecificStatesRepeatAfterFirstCall,
+                      const double *setSpecificStatesCallEnableBegin);
+
+void computeMonodomain(const float *parameters, double *algebraicsForTransfer, double *statesForTransfer,
+                       const float *elementLengths, double startTime, double timeStepWidthSplitting,
+                       int nTimeStepsSplitting, double dt0D, int nTimeSteps0D, double dt1D, int nTimeSteps1D,
+                       double prefactor, double valueForStimulatedPoint);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENDIHU_FMS_H */

@@ -1,0 +1,172 @@
+"""ctypes binding of the C ABI in include/opendihu_fms.h.
+
+The libraries live in ``opendihu_amd/lib``; loading fails loudly when the
+requested library was not built (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB_DIR, HH_LIBRARY
+
+FMS_IMPLICIT_EULER, FMS_CRANK_NICOLSON = 0, 1
+FMS_FP64, FMS_MIXED = 0, 1
+FMS_EXP_APPROXIMATE, FMS_EXP_LIBM = 0, 1
+FMS_THOMAS_SERIAL, FMS_THOMAS_PARTITIONED = 0, 1
+FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
+FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
+
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ucp = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class FmsConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_fibers", ctypes.c_int),
+        ("n_nodes", _ip),
+        ("element_lengths", _dp),
+        ("prefactor", _dp),
+        ("parameters", _dp),
+        ("initial_states", _dp),
+        ("scheme", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("exp_mode", ctypes.c_int),
+        ("thomas", ctypes.c_int),
+        ("kernels", ctypes.c_int),
+        ("value_for_stimulated_point", ctypes.c_double),
+    ]
+
+
+class FmsStimulation(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int),
+        ("stim_begin", _ip),
+        ("stim_end", _ip),
+        ("firing_rows", ctypes.c_int),
+        ("firing_cols", ctypes.c_int),
+        ("firing_events", _ucp),
+        ("motor_unit", _ip),
+        ("call_frequency", _dp),
+        ("repeat_after_first_call", _dp),
+        ("call_enable_begin", _dp),
+        ("jitter", _dp),
+        ("jitter_offset", _ip),
+        ("jitter_count", _ip),
+        ("last_check_time", _dp),
+        ("log_capacity", ctypes.c_int),
+        ("onset", _dp),
+        ("duration", _dp),
+        ("amplitude", _dp),
+        ("stim_param", ctypes.c_int),
+    ]
+
+
+class FmsReferenceDims(ctypes.Structure):
+    _fields_ = [
+        ("n_fibers_to_compute", ctypes.c_int),
+        ("n_instances_per_fiber", ctypes.c_int),
+        ("n_parameters_per_instance", ctypes.c_int),
+        ("firing_events_rows", ctypes.c_int),
+        ("firing_events_cols", ctypes.c_int),
+        ("frequency_jitter_cols", ctypes.c_int),
+        ("n_algebraics_for_transfer", ctypes.c_int),
+        ("n_states_for_transfer", ctypes.c_int),
+        ("scheme", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("exp_mode", ctypes.c_int),
+    ]
+
+
+# every symbol declared in include/opendihu_fms.h
+EXPORTED_SYMBOLS = [
+    "fms_model_info", "fms_model_defaults", "fms_create", "fms_destroy", "fms_set_stimulation",
+    "fms_advance", "fms_advance_schedule", "fms_synchronize", "fms_get_states", "fms_set_states",
+    "fms_get_algebraics", "fms_set_algebraics_for_transfer", "fms_total_nodes",
+    "fms_enable_spike_detection", "fms_get_spikes", "fms_get_stimulation_log", "fms_schedule",
+    "fms_last_error", "fms_reference_set_dims", "initializeArrays", "computeMonodomain",
+]
+
+_LIBS: dict[str, ctypes.CDLL] = {}
+
+
+def library_path(name: str = HH_LIBRARY) -> str:
+    return name if os.path.isabs(name) else os.path.join(LIB_DIR, name)
+
+
+def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
+    """Load (once) a built fibre library; raises if it does not exist."""
+    path = library_path(name)
+    if path in _LIBS:
+        return _LIBS[path]
+    if not os.path.exists(path):
+        raise RuntimeError(f"fibre library {path} is not built; run `python -m opendihu_amd.build` "
+                           "(or __graft_entry__.build()) first")
+    lib = ctypes.CDLL(path)
+    d, i, v = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    lib.fms_model_info.argtypes = [_ip, _ip, _ip, ctypes.c_char_p, i]
+    lib.fms_model_defaults.argtypes = [_dp, _dp]
+    lib.fms_create.argtypes = [ctypes.POINTER(FmsConfig), ctypes.POINTER(v)]
+    lib.fms_destroy.argtypes = [v]
+    lib.fms_set_stimulation.argtypes = [v, ctypes.POINTER(FmsStimulation)]
+    lib.fms_advance.argtypes = [v, d, d, i, d, i, d, i]
+    lib.fms_advance_schedule.argtypes = [v, _dp, i, i]
+    lib.fms_synchronize.argtypes = [v]
+    lib.fms_get_states.argtypes = [v, i, _dp]
+    lib.fms_set_states.argtypes = [v, i, _dp]
+    lib.fms_get_algebraics.argtypes = [v, _dp]
+    lib.fms_set_algebraics_for_transfer.argtypes = [v, _ip, i]
+    lib.fms_total_nodes.argtypes = [v, ctypes.POINTER(ctypes.c_longlong)]
+    lib.fms_enable_spike_detection.argtypes = [v, d]
+    lib.fms_get_spikes.argtypes = [v, _ip, _ip]
+    lib.fms_get_stimulation_log.argtypes = [v, _ip, _dp, _ip, i]
+    lib.fms_schedule.argtypes = [d, d, d, d, d, d, d, d, i, _dp]
+    lib.fms_last_error.argtypes = [ctypes.c_char_p, i]
+    lib.fms_reference_set_dims.argtypes = [ctypes.POINTER(FmsReferenceDims)]
+    lib.initializeArrays.argtypes = [_dp, _ip, _ip, ctypes.c_char_p, _dp, _ip, _ip, _dp, _dp, _dp, _dp]
+    lib.initializeArrays.restype = None
+    lib.computeMonodomain.argtypes = [ctypes.POINTER(ctypes.c_float), _dp, _dp, ctypes.POINTER(ctypes.c_float),
+                                      d, d, i, d, i, d, i, d, d]
+    lib.computeMonodomain.restype = None
+    for name_ in EXPORTED_SYMBOLS:
+        if name_ not in ("initializeArrays", "computeMonodomain"):
+            getattr(lib, name_).restype = ctypes.c_int
+    _LIBS[path] = lib
+    return lib
+
+
+def last_error(lib: ctypes.CDLL) -> str:
+    buf = ctypes.create_string_buffer(4096)
+    lib.fms_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(lib: ctypes.CDLL, rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error(lib)}")
+
+
+def as_ptr(a: np.ndarray, ptype):
+    return a.ctypes.data_as(ptype)
+
+
+def model_info(lib: ctypes.CDLL) -> tuple[int, int, int, str]:
+    ns, na, nparam = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    buf = ctypes.create_string_buffer(256)
+    check(lib, lib.fms_model_info(ctypes.byref(ns), ctypes.byref(na), ctypes.byref(nparam), buf, 256), "fms_model_info")
+    return ns.value, na.value, nparam.value, buf.value.decode()
+
+
+def schedule(lib: ctypes.CDLL, rc_end, rc_width, strang_end, strang_width, heun_end, heun_width, ie_end,
+             ie_width) -> np.ndarray:
+    """Per-call (startTime, dtSplit, nSplit, dt0D, n0D, dt1D, n1D) of a RepeatedCall run."""
+    n = lib.fms_schedule(rc_end, rc_width, strang_end, strang_width, heun_end, heun_width, ie_end, ie_width, 0, None)
+    if n < 0:
+        raise RuntimeError(last_error(lib))
+    out = np.zeros((max(n, 1), 7))
+    lib.fms_schedule(rc_end, rc_width, strang_end, strang_width, heun_end, heun_width, ie_end, ie_width, n,
+                     as_ptr(out, _dp))
+    return out[:n]

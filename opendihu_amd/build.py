@@ -1,0 +1,103 @@
+"""Ahead-of-time build of the fibre libraries for gfx950.
+
+One shared library per cell model (``libfms_<model>.so`` in ``opendihu_amd/lib``),
+the counterpart of the reference compiling one library per CellML file at run
+time (core/src/specialized_solver/fast_monodomain_solver/
+fast_monodomain_solver_gpu.tpp:12-206). The hand-written Hodgkin-Huxley model
+is always built; models generated from OpenCOR C files by
+``opendihu_amd.cellml`` are built with :func:`build_model_library`.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+CSRC = os.path.join(PKG_DIR, "csrc")
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+GEN_DIR = os.path.join(PKG_DIR, "_gen")
+INCLUDE = os.path.join(REPO_DIR, "include")
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FMS_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: every + - * / rounds as written, as in the reference's
+# g++ -std=c++17 build of its generated code (bit-exact parity).
+# -packed-fp32-ops: no packed FP32 instructions are generated.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+HH_LIBRARY = "libfms_hodgkin_huxley_1952.so"
+
+
+def _sources() -> list[str]:
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".hip")))
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build failed ({' '.join(cmd)}):\n{out}")
+
+
+def compile_library(out_name: str, model_header: str | None = None, model_type: str | None = None,
+                    extra_include: str | None = None) -> str:
+    """Compile fms_lib.hip for one model into ``LIB_DIR/out_name``."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    out = os.path.join(LIB_DIR, out_name)
+    cmd = [HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}"]
+    if extra_include:
+        cmd.append(f"-I{extra_include}")
+    if model_header:
+        cmd += [f'-DFMS_MODEL_HEADER="{model_header}"', f"-DFMS_MODEL_TYPE={model_type}"]
+    tmp = out + ".tmp"
+    cmd += ["-o", tmp, os.path.join(CSRC, "fms_lib.hip")]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_hh() -> str:
+    return compile_library(HH_LIBRARY)
+
+
+def build_model_library(model_c_file: str, parameters_used_as_algebraic=(), parameters_used_as_constant=(),
+                        name: str | None = None) -> str:
+    """Generate HIP code for an OpenCOR C model file and compile its library."""
+    from .cellml import generate_hip_model
+
+    os.makedirs(GEN_DIR, exist_ok=True)
+    with open(model_c_file) as f:
+        text = f.read()
+    base = name or os.path.splitext(os.path.basename(model_c_file))[0]
+    key = hashlib.sha1((text + repr(tuple(parameters_used_as_algebraic)) + repr(tuple(parameters_used_as_constant)))
+                       .encode()).hexdigest()[:8]
+    header_name = f"model_{base}_{key}.h"
+    type_name = f"fms::Model_{base}_{key}".replace("-", "_")
+    code = generate_hip_model(text, base, list(parameters_used_as_algebraic), list(parameters_used_as_constant),
+                              struct_name=type_name.split("::")[1])
+    with open(os.path.join(GEN_DIR, header_name), "w") as f:
+        f.write(code)
+    return compile_library(f"libfms_{base}.so", header_name, type_name, extra_include=GEN_DIR)
+
+
+def build_reference_models(reference_root: str) -> list[str]:
+    """Libraries for the CellML models shipped in the reference's test inputs."""
+    return []
+
+
+def main(argv=None) -> int:
+    print("building", build_hh(), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,237 @@
+// Fused multi-step split-step kernel: one workgroup per fibre runs many
+// Strang steps without leaving the chip.
+//
+//  * Vm of the fibre stays in LDS for the whole launch (<= 4096 nodes, 32 KiB).
+//  * Gates either stay in registers (RESIDENT: thread t owns nodes t, t+T, ...,
+//    NPT of them) or stream from HBM once per split step (large models).
+//  * The second reaction half of step s and the first half of step s+1 run in
+//    one pass over the nodes, so each split step reads and writes every gate
+//    once and Vm never touches HBM between launches.
+//  * Diffusion is the partitioned LDS solve (lds_thomas_part) or, for bit-exact
+//    runs, the reference's serial recurrence executed by thread 0 in LDS.
+//  * The stimulation state machine of the fibre runs on thread 0 in time order
+//    and hands the per-Heun-step flags to the workgroup through LDS.
+// Uneven fibre lengths: the host orders fibres longest first (LPT) so the
+// dispatcher fills CUs evenly; short fibres simply leave threads idle.
+#pragma once
+
+#include "fms_kernels.h"
+
+namespace fms {
+
+struct DevSpikes {
+  int enabled;
+  double threshold;
+  double *vm_prev;      // [N]
+  int *first_step;      // [N]
+  int *count;           // [N]
+  int step0;            // global index of the step before this launch's first step
+};
+
+struct DevTransfer {
+  double *alg;          // [n_idx*N] or nullptr
+  const int *idx;
+  int n_idx;
+};
+
+template <class M, class G, class E>
+__device__ __forceinline__ void fused_node_half(double &V, G *g, G *p, G pbase, const DevStim &st, bool inset,
+                                                const unsigned char *flags, double t0, double dt, int n0D,
+                                                double vstim, G *alg) {
+  for (int j = 0; j < n0D; j++) {
+    const int on = flags[j];
+    const bool clamp = st.mode == 1 && on && inset;
+    if (st.mode == 2 && inset && M::NP > 0) p[st.stim_param] = on ? (G)st.amplitude[0] : pbase;
+    heun_node<M, G, E>(V, g, p, t0 + j * dt, dt, clamp, vstim, (alg && j == n0D - 1) ? alg : nullptr);
+  }
+}
+
+// NPT > 0: gates resident in registers, NPT nodes per thread; NPT == 0: streamed
+template <class M, class G, class E, int T, int MMAX, int NPT>
+__global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *__restrict__ vm,
+                                             G *__restrict__ gates, const double *__restrict__ steps, int K,
+                                             int n0D, double vstim, int ie, int serial, const int *__restrict__ order,
+                                             DevSpikes spk, DevTransfer tr) {
+  extern __shared__ double sh[];
+  const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
+  const int tid = threadIdx.x;
+  const long long N = fb.n_total;
+  const int off = fb.fiber_off[f];
+  const int n = fb.fiber_off[f + 1] - off;
+  const double *h = fb.elem_len + off - f;
+  const double pf = fb.prefactor[f];
+  double *red = sh;                       // 16*T doubles: PCR system, or serial cA/dA
+  double *lv = sh + 16 * T;               // n doubles: Vm
+  unsigned char *sflags = (unsigned char *)(lv + n);   // 2*n0D flags
+
+  for (int i = tid; i < n; i += T) lv[i] = vm[off + i];
+
+  G p[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+  for (int k = 0; k < M::NP; k++) p[k] = (G)fb.params[(long long)f * M::NP + k];
+  const G pbase = M::NP > 0 ? p[st.stim_param] : G(0);
+  const int sb = fb.stim_begin[f], se = fb.stim_end[f];
+  // the current-pulse amplitude is read through a per-fibre view
+  DevStim stf = st;
+  if (st.mode == 2) stf.amplitude = st.amplitude + f;
+
+  G greg[NPT > 0 ? NPT : 1][M::NS - 1];
+  double vprev[NPT > 0 ? NPT : 1];
+  int sfirst[NPT > 0 ? NPT : 1], scount[NPT > 0 ? NPT : 1];
+  if (NPT > 0) {
+#pragma unroll
+    for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+      const int i = tid + q * T;
+      if (i < n) {
+#pragma unroll
+        for (int k = 0; k < M::NS - 1; k++) greg[q][k] = gates[(long long)k * N + off + i];
+        if (spk.enabled) {
+          vprev[q] = spk.vm_prev[off + i];
+          sfirst[q] = spk.first_step[off + i];
+          scount[q] = spk.count[off + i];
+        }
+      }
+    }
+  }
+
+  // flags of the first half step of step 0
+  if (tid == 0)
+    for (int j = 0; j < n0D; j++) sflags[j] = (unsigned char)stim_flag(st, f, steps[0] + j * steps[2]);
+  __syncthreads();
+
+  // pass 0: first reaction half of step 0
+  {
+    const double t0 = steps[0], dt0 = steps[2];
+    if (NPT > 0) {
+#pragma unroll
+      for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+        const int i = tid + q * T;
+        if (i < n) {
+          double V = lv[i];
+          fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim,
+                                   nullptr);
+          lv[i] = V;
+        }
+      }
+    } else {
+      for (int i = tid; i < n; i += T) {
+        G g[M::NS - 1];
+#pragma unroll
+        for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + off + i];
+        double V = lv[i];
+        fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim, nullptr);
+        lv[i] = V;
+#pragma unroll
+        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = g[k];
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int s = 0; s < K; s++) {
+    const double *st_s = steps + 4 * s;
+    const double t_mid = st_s[0] + 0.5 * st_s[1];
+    const double dt0 = st_s[2], dt1 = st_s[3];
+    const bool more = s + 1 < K;
+
+    // diffusion (one solve per split step, as the reference)
+    if (serial) {
+      if (tid == 0) thomas_serial_fiber(lv, n, h, pf, dt1, ie != 0, red, red + n);
+      __syncthreads();
+    } else {
+      lds_thomas_part<T, MMAX>(lv, n, h, pf, dt1, ie != 0, red, tid);
+    }
+
+    // flags: second half of step s, then first half of step s+1
+    if (tid == 0) {
+      for (int j = 0; j < n0D; j++) sflags[j] = (unsigned char)stim_flag(st, f, t_mid + j * dt0);
+      if (more) {
+        const double t1 = st_s[4], dt0n = st_s[6];
+        for (int j = 0; j < n0D; j++) sflags[n0D + j] = (unsigned char)stim_flag(st, f, t1 + j * dt0n);
+      }
+    }
+    __syncthreads();
+
+    const bool store_alg = tr.alg != nullptr && !more;
+    auto node = [&](int i, double &V, G *g, double &vp, int &sf, int &sc) {
+      G alg[M::NA];
+      fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim,
+                               store_alg ? alg : nullptr);
+      if (store_alg) {
+        for (int k = 0; k < tr.n_idx; k++) {
+          const int a = tr.idx[k];
+          double v = 0;
+#pragma unroll
+          for (int q = 0; q < M::NA; q++)
+            if (q == a) v = (double)alg[q];
+          tr.alg[(long long)k * N + off + i] = v;
+        }
+      }
+      if (spk.enabled) {
+        if (vp < spk.threshold && V >= spk.threshold) {
+          sc++;
+          if (sf < 0) sf = spk.step0 + s + 1;
+        }
+        vp = V;
+      }
+      if (more)
+        fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D,
+                                 vstim, nullptr);
+    };
+
+    if (NPT > 0) {
+#pragma unroll
+      for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+        const int i = tid + q * T;
+        if (i < n) {
+          double V = lv[i];
+          node(i, V, greg[q], vprev[q], sfirst[q], scount[q]);
+          lv[i] = V;
+        }
+      }
+    } else {
+      for (int i = tid; i < n; i += T) {
+        G g[M::NS - 1];
+#pragma unroll
+        for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + off + i];
+        double V = lv[i];
+        double vp = 0;
+        int sf = 0, sc = 0;
+        if (spk.enabled) {
+          vp = spk.vm_prev[off + i];
+          sf = spk.first_step[off + i];
+          sc = spk.count[off + i];
+        }
+        node(i, V, g, vp, sf, sc);
+        lv[i] = V;
+        if (spk.enabled) {
+          spk.vm_prev[off + i] = vp;
+          spk.first_step[off + i] = sf;
+          spk.count[off + i] = sc;
+        }
+#pragma unroll
+        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = g[k];
+      }
+    }
+    __syncthreads();
+  }
+
+  for (int i = tid; i < n; i += T) vm[off + i] = lv[i];
+  if (NPT > 0) {
+#pragma unroll
+    for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+      const int i = tid + q * T;
+      if (i < n) {
+#pragma unroll
+        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = greg[q][k];
+        if (spk.enabled) {
+          spk.vm_prev[off + i] = vprev[q];
+          spk.first_step[off + i] = sfirst[q];
+          spk.count[off + i] = scount[q];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace fms

@@ -1,0 +1,417 @@
+// HIP kernels of the FastMonodomainSolver split step for CDNA4 (gfx950).
+//
+// One Strang step (core/src/specialized_solver/fast_monodomain_solver/
+// fast_monodomain_solver_compute.tpp:110-126) is
+//     reaction half (Heun, t)  ->  diffusion (one tridiagonal solve, dt1D)
+//     ->  reaction half (Heun, t + dt/2)
+// Storage is state-major (SoA): Vm is one FP64 array over all nodes of all
+// fibres (fibres concatenated), the remaining NS-1 states are NS-1 arrays of
+// the gate type G (double, or float for the mixed-precision form).
+//
+// Kernels here:
+//   k_stim_flags      one thread per fibre: stimulation state machine per Heun step
+//   k_reaction        one node per lane, coalesced SoA streaming, n0D Heun steps
+//   k_thomas_serial   one lane per fibre, the reference's Thomas recurrence (bit-exact)
+//   k_thomas_part     one workgroup per fibre, partitioned Thomas + PCR in LDS
+//   k_fused           one workgroup per fibre, many split steps; Vm lives in LDS,
+//                     gates in registers (RESIDENT) or streamed from HBM
+#pragma once
+
+#include "fms_math.h"
+#include <stdint.h>
+
+namespace fms {
+
+// ----------------------------------------------------------------------------
+// device views
+// ----------------------------------------------------------------------------
+
+struct DevFibers {
+  int n_fibers;
+  long long n_total;
+  const int *fiber_off;      // [F+1] first node of each fibre
+  const int *node_fiber;     // [N] fibre of each node
+  const double *elem_len;    // [N-F] element lengths, fibre f starts at fiber_off[f]-f
+  const double *prefactor;   // [F] sigma/(Am*Cm)
+  const double *params;      // [F*NP]
+  const int *stim_begin;     // [F] first stimulated node (fibre-local)
+  const int *stim_end;       // [F] one past the last stimulated node
+};
+
+struct DevStim {
+  int mode;                  // 0 none, 1 firing table (clamp Vm), 2 current pulse
+  // firing table, reference semantics (fast_monodomain_solver_compute.tpp:509-627)
+  int fe_rows, fe_cols;
+  const unsigned char *firing_events;  // [rows*cols]
+  const int *motor_unit;
+  const double *freq, *repeat, *enable_begin;
+  const double *jitter;
+  const int *jitter_off, *jitter_cnt;
+  double *last_check, *cur_jitter;
+  int *jitter_idx;
+  unsigned char *currently, *has_been;
+  int *log_count;            // stimulation log (StimulationLogging::logStimulationBegin)
+  double *log_time;
+  int *log_fiber;
+  int log_cap;
+  // current pulse (extension)
+  const double *onset, *duration, *amplitude;
+  int stim_param;
+};
+
+// ----------------------------------------------------------------------------
+// Heun step of one node, reference form (03_generator_vc.cpp:1017-1130):
+//   y* = y + dt f(y); [clamp y*_0]; y += 0.5 dt (f(y) + f(y*)); [clamp y_0]
+// Vm is FP64 in both precisions; the gates and the RHS arithmetic use G.
+// ----------------------------------------------------------------------------
+template <class M, class G, class E>
+FMS_HD void heun_node(double &V, G *g, const G *p, double t, double dt, bool clamp, double vstim,
+                      G *alg_out = nullptr) {
+  G y[M::NS], r[M::NS], ys[M::NS], rs[M::NS], a[M::NA], as[M::NA];
+  y[0] = (G)V;
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) y[k] = g[k - 1];
+  M::template rhs<G, E>(y, p, t, r, a);
+  double Vs = V + dt * (double)r[0];
+  const G gdt = (G)dt;
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) ys[k] = y[k] + gdt * r[k];
+  if (clamp) Vs = vstim;
+  ys[0] = (G)Vs;
+  M::template rhs<G, E>(ys, p, t, rs, as);
+  V += 0.5 * dt * ((double)r[0] + (double)rs[0]);
+  const G ghdt = (G)(0.5 * dt);
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) g[k - 1] = y[k] + ghdt * (r[k] + rs[k]);
+  if (clamp) V = vstim;
+  if (alg_out) {
+#pragma unroll
+    for (int k = 0; k < M::NA; k++) alg_out[k] = as[k];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// stimulation state machine of one fibre at time t
+// (fast_monodomain_solver_compute.tpp:529-623)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ int stim_check(const DevStim &s, int f, double t) {
+  const double freq = s.freq[f];
+  const double repeat = s.repeat[f];
+  const double begin = s.enable_begin[f];
+  int check = 0;
+  if (t >= s.last_check[f] + 1. / (freq + s.cur_jitter[f]) && t >= begin - 1e-13) {
+    check = 1;
+    if (repeat != 0 && t - (s.last_check[f] + 1. / (freq + s.cur_jitter[f])) > repeat) {
+      s.last_check[f] += 1. / (freq + s.cur_jitter[f]);
+      double jf = 0.0;
+      if (s.jitter_cnt[f] > 0) jf = s.jitter[s.jitter_off[f] + s.jitter_idx[f] % s.jitter_cnt[f]];
+      s.cur_jitter[f] = jf * freq;
+      s.jitter_idx[f]++;
+      check = 0;
+    }
+  }
+  const int idx = (int)round(t * freq);
+  const unsigned long row = (unsigned long)(long)idx % (unsigned long)s.fe_rows;
+  const int col = s.motor_unit[f] % s.fe_cols;
+  const int stim = check && s.firing_events[row * s.fe_cols + col];
+  if (stim) {
+    s.has_been[f] = 1;
+    if (!s.currently[f]) {
+      s.currently[f] = 1;
+      if (s.log_count) {
+        const int k = atomicAdd(s.log_count, 1);
+        if (k < s.log_cap) {
+          s.log_time[k] = t;
+          s.log_fiber[k] = f;
+        }
+      }
+    }
+  } else {
+    s.currently[f] = 0;
+  }
+  return stim;
+}
+
+__device__ __forceinline__ int current_on(const DevStim &s, int f, double t) {
+  return t >= s.onset[f] - 1e-13 && t < s.onset[f] + s.duration[f] - 1e-13;
+}
+
+__device__ __forceinline__ int stim_flag(const DevStim &s, int f, double t) {
+  if (s.mode == 1) return stim_check(s, f, t);
+  if (s.mode == 2) return current_on(s, f, t);
+  return 0;
+}
+
+// one thread per fibre: flags[f*n0D + j] for the n0D Heun steps of a half step
+__global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, double t0, double dt, int n0D) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fibers) return;
+  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = (unsigned char)stim_flag(st, f, t0 + j * dt);
+}
+
+// ----------------------------------------------------------------------------
+// reaction: one node per lane, n0D Heun steps, SoA coalesced streaming
+// ----------------------------------------------------------------------------
+template <class M, class G, class E>
+__global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, double *__restrict__ vm,
+                                                  G *__restrict__ gates, const unsigned char *__restrict__ flags,
+                                                  double t0, double dt, int n0D, double vstim,
+                                                  double *__restrict__ alg_transfer, const int *__restrict__ alg_idx,
+                                                  int n_alg_transfer) {
+  const long long N = fb.n_total;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int f = fb.node_fiber[i];
+  const int li = (int)(i - fb.fiber_off[f]);
+  G p[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+  for (int k = 0; k < M::NP; k++) p[k] = (G)fb.params[(long long)f * M::NP + k];
+  const bool inset = li >= fb.stim_begin[f] && li < fb.stim_end[f];
+  double V = vm[i];
+  G g[M::NS - 1];
+#pragma unroll
+  for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
+  const G pbase = M::NP > 0 ? p[st.stim_param] : G(0);
+  for (int j = 0; j < n0D; j++) {
+    const int on = flags[(long long)f * n0D + j];
+    const bool clamp = st.mode == 1 && on && inset;
+    if (st.mode == 2 && inset && M::NP > 0) p[st.stim_param] = on ? (G)st.amplitude[f] : pbase;
+    G alg[M::NA];
+    const bool store = alg_transfer != nullptr && j == n0D - 1;
+    heun_node<M, G, E>(V, g, p, t0 + j * dt, dt, clamp, vstim, store ? alg : nullptr);
+    if (store) {
+      for (int k = 0; k < n_alg_transfer; k++) {
+        const int a = alg_idx[k];
+        double v = 0;
+#pragma unroll
+        for (int q = 0; q < M::NA; q++)
+          if (q == a) v = (double)alg[q];
+        alg_transfer[(long long)k * N + i] = v;
+      }
+    }
+  }
+  vm[i] = V;
+#pragma unroll
+  for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + i] = g[k];
+}
+
+// ----------------------------------------------------------------------------
+// Row i of the split-step diffusion system for one fibre, exactly as
+// fast_monodomain_solver_compute.tpp:336-429 assembles it (consistent P1 mass):
+//   IE: (K - M/dt) x = -(M/dt) u        CN: (K/2 - M/dt) x = (-K/2 - M/dt) u
+// u points at the fibre's Vm (global or LDS), h at its element lengths.
+// ----------------------------------------------------------------------------
+FMS_HD void assemble_row(int i, int n, const double *u, const double *h, double prefactor, double dt, bool ie,
+                         double &a, double &b, double &c, double &d) {
+  a = 0; b = 0; c = 0; d = 0;
+  const double u_center = u[i];
+  if (i > 0) {
+    const double u_previous = u[i - 1];
+    const double h_left = h[i - 1];
+    const double k_left = 1. / h_left * (1) * prefactor;
+    const double m_left = h_left * 1. / 6;
+    const double k_right = 1. / h_left * (-1) * prefactor;
+    const double m_right = h_left * 1. / 3;
+    if (ie) {
+      a = (k_left - 1 / dt * m_left);
+      b += (k_right - 1 / dt * m_right);
+      d += (-1 / dt * m_left) * u_previous + (-1 / dt * m_right) * u_center;
+    } else {
+      a = (k_left / 2. - 1 / dt * m_left);
+      b += (k_right / 2. - 1 / dt * m_right);
+      d += (-k_left / 2. - 1 / dt * m_left) * u_previous + (-k_right / 2. - 1 / dt * m_right) * u_center;
+    }
+  }
+  if (i < n - 1) {
+    const double u_next = u[i + 1];
+    const double h_right = h[i];
+    const double k_right = 1. / h_right * (1) * prefactor;
+    const double m_right = h_right * 1. / 6;
+    const double k_left = 1. / h_right * (-1) * prefactor;
+    const double m_left = h_right * 1. / 3;
+    if (ie) {
+      c = (k_right - 1 / dt * m_right);
+      b += (k_left - 1 / dt * m_left);
+      d += (-1 / dt * m_left) * u_center + (-1 / dt * m_right) * u_next;
+    } else {
+      c = (k_right / 2. - 1 / dt * m_right);
+      b += (k_left / 2. - 1 / dt * m_left);
+      d += (-k_left / 2. - 1 / dt * m_left) * u_center + (-k_right / 2. - 1 / dt * m_right) * u_next;
+    }
+  }
+}
+
+// serial Thomas on one fibre, the reference recurrence (compute.tpp:431-486);
+// cA/dA are n-long scratch arrays
+FMS_HD void thomas_serial_fiber(double *u, int n, const double *h, double prefactor, double dt, bool ie,
+                                double *cA, double *dA) {
+  for (int i = 0; i < n; i++) {
+    double a, b, c, d;
+    assemble_row(i, n, u, h, prefactor, dt, ie, a, b, c, d);
+    if (i == 0) {
+      cA[i] = c / b;
+      dA[i] = d / b;
+    } else {
+      if (i != n - 1) cA[i] = c / (b - cA[i - 1] * a);
+      dA[i] = (d - dA[i - 1] * a) / (b - cA[i - 1] * a);
+    }
+  }
+  u[n - 1] = dA[n - 1];
+  double previous = dA[n - 1];
+  for (int i = n - 2; i >= 0; i--) {
+    const double r = dA[i] - cA[i] * previous;
+    u[i] = r;
+    previous = r;
+  }
+}
+
+// one lane per fibre: bit-exact reference recurrence
+__global__ __launch_bounds__(64) void k_thomas_serial(DevFibers fb, double *vm, double *cA, double *dA, double dt,
+                                                      int ie) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= fb.n_fibers) return;
+  const int off = fb.fiber_off[f];
+  const int n = fb.fiber_off[f + 1] - off;
+  thomas_serial_fiber(vm + off, n, fb.elem_len + off - f, fb.prefactor[f], dt, ie != 0, cA + off, dA + off);
+}
+
+// ----------------------------------------------------------------------------
+// Partitioned tridiagonal solve of one fibre by a group of T threads, Vm in LDS.
+//
+// Each of P <= T threads owns a contiguous chunk of >= 2 rows. Within a chunk a
+// forward and a backward elimination express every row through the chunk's
+// first and last unknowns; the first/last rows of all chunks form a tridiagonal
+// system of 2T rows (identity rows for idle threads), solved by cyclic
+// reduction (PCR) in LDS; then every thread recovers its interior unknowns.
+// Rows are assembled exactly as the reference does; only the elimination order
+// differs from its serial recurrence. The matrix is strictly diagonally
+// dominant for every dt (mass term), so no pivoting is needed.
+//
+// lv: n doubles of Vm in LDS (overwritten with the solution)
+// red: 8*2T doubles of LDS scratch
+// Requires n >= 2 and ceil(n / min(T, n/2)) <= MMAX. Ends with a barrier.
+// ----------------------------------------------------------------------------
+template <int T, int MMAX>
+__device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double *h, double prefactor, double dt,
+                                                bool ie, double *red, int tid) {
+  constexpr int R = 2 * T;
+  const int P = n / 2 < T ? n / 2 : T;
+  double aa[MMAX], cc[MMAX], dd[MMAX];
+  int lo = 0, cnt = 0;
+  if (tid < P) {
+    lo = (int)(((long long)tid * n) / P);
+    cnt = (int)(((long long)(tid + 1) * n) / P) - lo;
+  }
+  double fa = 0, fc = 0, fd = 0, la = 0, lc = 0, ld = 0;
+  if (cnt > 0) {
+#pragma unroll
+    for (int k = 0; k < MMAX; k++) {
+      if (k < cnt) {
+        double a, b, c, d;
+        assemble_row(lo + k, n, lv, h, prefactor, dt, ie, a, b, c, d);
+        if (k < 2) {
+          aa[k] = a / b;
+          cc[k] = c / b;
+          dd[k] = d / b;
+        } else {
+          const double r = 1.0 / (b - a * cc[k - 1]);
+          dd[k] = r * (d - a * dd[k - 1]);
+          aa[k] = -r * a * aa[k - 1];
+          cc[k] = r * c;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = MMAX - 3; k >= 1; k--) {
+      if (k <= cnt - 3) {
+        dd[k] -= cc[k] * dd[k + 1];
+        aa[k] -= cc[k] * aa[k + 1];
+        cc[k] = -cc[k] * cc[k + 1];
+      }
+    }
+    if (cnt >= 3) {
+      const double r = 1.0 / (1.0 - cc[0] * aa[1]);
+      dd[0] = r * (dd[0] - cc[0] * dd[1]);
+      aa[0] = r * aa[0];
+      cc[0] = -r * cc[0] * cc[1];
+    }
+    fa = aa[0]; fc = cc[0]; fd = dd[0];
+#pragma unroll
+    for (int k = 0; k < MMAX; k++)
+      if (k == cnt - 1) { la = aa[k]; lc = cc[k]; ld = dd[k]; }
+  }
+  double *A0 = red, *B0 = red + R, *C0 = red + 2 * R, *D0 = red + 3 * R;
+  double *A1 = red + 4 * R, *B1 = red + 5 * R, *C1 = red + 6 * R, *D1 = red + 7 * R;
+  A0[2 * tid] = fa; B0[2 * tid] = 1.0; C0[2 * tid] = fc; D0[2 * tid] = fd;
+  A0[2 * tid + 1] = la; B0[2 * tid + 1] = 1.0; C0[2 * tid + 1] = lc; D0[2 * tid + 1] = ld;
+  __syncthreads();
+  for (int s = 1; s < R; s <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int i = tid + q * T;
+      const int im = i - s, ip = i + s;
+      double na = 0, nc = 0, nb = B0[i], nd = D0[i];
+      if (im >= 0) {
+        const double al = -A0[i] / B0[im];
+        na = al * A0[im];
+        nb += al * C0[im];
+        nd += al * D0[im];
+      }
+      if (ip < R) {
+        const double ga = -C0[i] / B0[ip];
+        nc = ga * C0[ip];
+        nb += ga * A0[ip];
+        nd += ga * D0[ip];
+      }
+      A1[i] = na; B1[i] = nb; C1[i] = nc; D1[i] = nd;
+    }
+    __syncthreads();
+    double *t;
+    t = A0; A0 = A1; A1 = t;
+    t = B0; B0 = B1; B1 = t;
+    t = C0; C0 = C1; C1 = t;
+    t = D0; D0 = D1; D1 = t;
+  }
+  if (cnt > 0) {
+    const double xf = D0[2 * tid] / B0[2 * tid];
+    const double xl = D0[2 * tid + 1] / B0[2 * tid + 1];
+    lv[lo] = xf;
+    lv[lo + cnt - 1] = xl;
+#pragma unroll
+    for (int k = 1; k < MMAX - 1; k++)
+      if (k < cnt - 1) lv[lo + k] = dd[k] - aa[k] * xf - cc[k] * xl;
+  }
+  __syncthreads();
+}
+
+// one workgroup per fibre: load Vm to LDS, partitioned solve, store
+template <int T, int MMAX>
+__global__ __launch_bounds__(T) void k_thomas_part(DevFibers fb, double *vm, double dt, int ie) {
+  extern __shared__ double sh[];
+  const int f = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int off = fb.fiber_off[f];
+  const int n = fb.fiber_off[f + 1] - off;
+  double *red = sh;
+  double *lv = sh + 8 * 2 * T;
+  for (int i = tid; i < n; i += T) lv[i] = vm[off + i];
+  __syncthreads();
+  lds_thomas_part<T, MMAX>(lv, n, fb.elem_len + off - f, fb.prefactor[f], dt, ie != 0, red, tid);
+  for (int i = tid; i < n; i += T) vm[off + i] = lv[i];
+}
+
+// spike detector: upward crossings of the threshold after each split step
+__global__ void k_spikes(long long N, const double *vm, double *vm_prev, int *first_step, int *count, double th,
+                         int step) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double v = vm[i];
+  const double p = vm_prev[i];
+  if (p < th && v >= th) {
+    count[i]++;
+    if (first_step[i] < 0) first_step[i] = step;
+  }
+  vm_prev[i] = v;
+}
+
+}  // namespace fms

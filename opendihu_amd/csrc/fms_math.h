@@ -1,0 +1,76 @@
+// Device/host math shared by every cell model of the fibre solver.
+//
+// The helpers follow the code the reference's CellML generator emits for its
+// FastMonodomainSolver kernels (core/src/cellml/source_code_generator/
+// 03_generator_vc.cpp:304-617), written here as templates over the arithmetic
+// type so the same model source serves the FP64 and the FP32-state paths:
+//   - exponential: (1 + x/1024)^1024 by ten squarings (03_generator_vc.cpp:385-407)
+//   - logarithmic: Taylor about 1, 3 or 9 + one Newton step (03_generator_vc.cpp:431-490)
+//   - powN / powReciprocalN: recursive products (03_generator_vc.cpp:541-611)
+// Libraries are compiled with -ffp-contract=off for the bit-exact variants, so
+// each + - * / below rounds exactly as the reference's g++ -std=c++17 build does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define FMS_HD __host__ __device__ __forceinline__
+
+namespace fms {
+
+// exp/log policy: the reference's default approximation (approximateExponentialFunction=True)
+struct ExpApprox {
+  template <class T> static FMS_HD T exp(T x) {
+    x = T(1.0) + x / T(1024.);
+#pragma unroll
+    for (int i = 0; i < 10; i++) x *= x;
+    return x;
+  }
+  template <class T> static FMS_HD T log(T x) {
+    T t1 = x - T(1);
+    T t1_2 = t1 * t1;
+    T t1_4 = t1_2 * t1_2;
+    T t1_6 = t1_4 * t1_2;
+    T result1 = t1 - T(0.5) * t1_2 + T(1. / 3) * t1_2 * t1 - T(0.25) * t1_4 + T(0.2) * t1_4 * t1 - T(1. / 6) * t1_6;
+    T t3 = x - T(3);
+    T t3_2 = t3 * t3;
+    T t3_4 = t3_2 * t3_2;
+    T result3 = T(1.0986122886681098) + T(1. / 3) * t3 - T(0.05555555555555555) * t3_2 +
+                T(0.012345679012345678) * t3_2 * t3 - T(0.0030864197530864196) * t3_4;
+    T t9 = x - T(9);
+    T t9_2 = t9 * t9;
+    T t9_4 = t9_2 * t9_2;
+    T result9 = T(2.1972245773362196) + T(1. / 9) * t9 - T(0.006172839506172839) * t9_2 +
+                T(0.0004572473708276177) * t9_2 * t9 - T(3.8103947568968146e-05) * t9_4;
+    T result = result1;
+    if (x >= T(2) && x < T(6)) result = result3;
+    if (x >= T(6)) result = result9;
+    result -= (T(1) - x / exp(result));
+    return result;
+  }
+};
+
+// exp/log policy: the device math library (approximateExponentialFunction=False)
+struct ExpLibm {
+  static FMS_HD double exp(double x) { return ::exp(x); }
+  static FMS_HD float exp(float x) { return ::expf(x); }
+  static FMS_HD double log(double x) { return ::log(x); }
+  static FMS_HD float log(float x) { return ::logf(x); }
+};
+
+template <class T> FMS_HD T pow2(T x) { return x * x; }
+template <class T> FMS_HD T pow3(T x) { return x * (pow2(x)); }
+template <class T> FMS_HD T pow4(T x) { return pow2(pow2(x)); }
+template <class T> FMS_HD T pow5(T x) { return pow2(pow3(x)); }
+template <class T> FMS_HD T pow6(T x) { return pow3(pow3(x)); }
+template <class T> FMS_HD T pow7(T x) { return pow3(pow4(x)); }
+template <class T> FMS_HD T pow8(T x) { return pow4(pow4(x)); }
+template <class T> FMS_HD T powReciprocal1(T x) { return T(1.) / (x); }
+template <class T> FMS_HD T powReciprocal2(T x) { return T(1.) / (pow2(x)); }
+template <class T> FMS_HD T powReciprocal3(T x) { return T(1.) / (x * (pow2(x))); }
+template <class T> FMS_HD T powReciprocal4(T x) { return T(1.) / (pow2(pow2(x))); }
+
+// ternary of the CellML C subset: both branches are values, pick one
+template <class T> FMS_HD T select(bool c, T a, T b) { return c ? a : b; }
+
+}  // namespace fms

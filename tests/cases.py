@@ -1,0 +1,56 @@
+"""Seeded workloads shared by the parity tests, smoke() and bench.py.
+
+Sizes follow the fibre configurations the GPU path is measured on:
+  c1     HH, 1 fibre x 1001 nodes (1000 elements, 10 cm), Am=500, Cm=1,
+         sigma=3.828, CN, dt=1e-3 ms, 1200 uA/cm^2 on the central 3 nodes for 0.1 ms
+  uneven HH, fibres of U{lo..hi} nodes, per-fibre sigma ~ U(3.0,4.5),
+         stimulus node ~ U{0..n-1} and onset ~ U(0,2) ms (current pulses)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+AM, CM, SIGMA = 500.0, 1.0, 3.828
+
+
+def c1(n_elements=1000, length_cm=10.0):
+    n = n_elements + 1
+    h = np.full(n_elements, length_cm / n_elements)
+    pf = SIGMA / (AM * CM)
+    mid = n // 2
+    stim = dict(stim_begin=mid - 1, stim_end=mid + 2, onset=0.0, duration=0.1, amplitude=1200.0, stim_param=0)
+    return dict(n_nodes=np.array([n], np.int32), element_lengths=h, prefactor=np.array([pf])), stim
+
+
+def uneven(n_fibers, lo=400, hi=2000, seed=3, h=0.01):
+    rng = np.random.default_rng(seed)
+    n = rng.integers(lo, hi + 1, size=n_fibers).astype(np.int32)
+    sigma = np.random.default_rng(seed + 1).uniform(3.0, 4.5, size=n_fibers)
+    r5 = np.random.default_rng(seed + 2)
+    nmj = np.array([r5.integers(0, k) for k in n], np.int32)
+    onset = r5.uniform(0.0, 2.0, size=n_fibers)
+    elem = np.concatenate([np.full(k - 1, h) for k in n])
+    pf = sigma / (AM * CM)
+    sb = np.maximum(nmj - 1, 0).astype(np.int32)
+    se = np.minimum(nmj + 2, n).astype(np.int32)
+    stim = dict(stim_begin=sb, stim_end=se, onset=onset, duration=np.full(n_fibers, 0.1),
+                amplitude=np.full(n_fibers, 1200.0), stim_param=0)
+    return dict(n_nodes=n, element_lengths=elem, prefactor=pf), stim
+
+
+def vc_lane_stim_range(offsets, nmj, n_nodes, width=4):
+    """Nodes the reference's Vc path clamps: lanes 0..min(3,W)-1 of the W-aligned
+    point buffer holding the NMJ (fast_monodomain_solver_compute.tpp:161-166,
+    03_generator_vc.cpp:1032-1038); clipped to the fibre."""
+    offsets = np.asarray(offsets)
+    nmj = np.asarray(nmj)
+    base = width * ((offsets + nmj) // width) - offsets
+    begin = np.clip(base, 0, n_nodes)
+    end = np.clip(base + min(3, width), 0, n_nodes)
+    return begin.astype(np.int32), end.astype(np.int32)
+
+
+def firing_table(n_rows, n_mu, rate_per_ms, dt_row, seed):
+    """0/1 table, rows = sampling steps of width dt_row, columns = motor units (Poisson firing)."""
+    rng = np.random.default_rng(seed)
+    return (rng.random((n_rows, n_mu)) < rate_per_ms * dt_row).astype(np.uint8)

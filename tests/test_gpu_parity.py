@@ -1,0 +1,206 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, on MI355X.
+
+Bars (FP64):
+  * thomas="serial": bit-exact in every state (same operations in the same order)
+  * thomas="partitioned": max |dVm| / max|Vm| <= 1e-10 after N split steps
+  * spike times (first upward crossing of 0 mV, per node) equal at step resolution
+Mixed precision (FP32 gates, FP64 Vm/Thomas) is compared with the FP64 oracle
+with the error stated in the assertion (spike-time error <= 1 step).
+"""
+import numpy as np
+import pytest
+
+from cases import c1, uneven, vc_lane_stim_range, firing_table
+from oracle_binding import OracleBatch
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-10
+
+
+def _gpu_batch(mesh, **kw):
+    from opendihu_amd.fibers import FiberBatch
+    return FiberBatch(**mesh, **kw)
+
+
+def _rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b))))
+
+
+def _run_pair(mesh, stim, n_steps, dt=1e-3, scheme="CN", gpu_kw=None, oracle_exp="approx", spikes=True,
+              params=None):
+    gpu_kw = dict(gpu_kw or {})
+    gpu_kw.setdefault("exp_mode", oracle_exp)
+    g = _gpu_batch(mesh, scheme=scheme, parameters=params, **gpu_kw)
+    o = OracleBatch(**mesh, scheme=scheme, exp_mode=oracle_exp, parameters=params, threads=8)
+    for b in (g, o):
+        b.set_current_stimulation(**stim)
+        if spikes:
+            b.enable_spike_detection(0.0)
+        b.advance(0.0, dt, n_steps)
+    return g, o
+
+
+@pytest.mark.parametrize("kernels", ["split", "fused"])
+def test_c1_serial_thomas_bit_exact(kernels):
+    mesh, stim = c1()
+    g, o = _run_pair(mesh, stim, 3000, gpu_kw=dict(thomas="serial", kernels=kernels))
+    for k in range(4):
+        np.testing.assert_array_equal(g.get_state(k), o.get_state(k), err_msg=f"state {k} ({kernels})")
+    np.testing.assert_array_equal(g.spikes()[0], o.spikes()[0])
+
+
+@pytest.mark.parametrize("kernels", ["split", "fused"])
+def test_c1_partitioned_thomas(kernels):
+    mesh, stim = c1()
+    g, o = _run_pair(mesh, stim, 10000, gpu_kw=dict(thomas="partitioned", kernels=kernels))
+    err = _rel_err(g.vm(), o.vm())
+    assert err <= REL_TOL, f"Vm rel err {err:.3e} ({kernels})"
+    gf, gc = g.spikes()
+    of, oc = o.spikes()
+    assert (of > 0).sum() > 900, "the action potential must cross the fibre"
+    np.testing.assert_array_equal(gf, of)
+    np.testing.assert_array_equal(gc, oc)
+
+
+def test_c1_libm_exp():
+    mesh, stim = c1()
+    g, o = _run_pair(mesh, stim, 4000, oracle_exp="libm", gpu_kw=dict(thomas="serial", kernels="fused"))
+    err = _rel_err(g.vm(), o.vm())
+    assert err <= REL_TOL, f"Vm rel err {err:.3e} with device exp vs libm exp"
+    np.testing.assert_array_equal(g.spikes()[0], o.spikes()[0])
+
+
+@pytest.mark.parametrize("kernels,scheme", [("fused", "CN"), ("split", "IE"), ("fused", "IE")])
+def test_uneven_batch(kernels, scheme):
+    mesh, stim = uneven(48, seed=11)
+    g, o = _run_pair(mesh, stim, 2500, scheme=scheme, gpu_kw=dict(kernels=kernels))
+    err = _rel_err(g.vm(), o.vm())
+    assert err <= REL_TOL, f"Vm rel err {err:.3e}"
+    np.testing.assert_array_equal(g.spikes()[0], o.spikes()[0])
+    for k in range(1, 4):
+        assert _rel_err(g.get_state(k), o.get_state(k)) <= REL_TOL
+
+
+def test_per_fibre_parameters():
+    mesh, stim = uneven(16, lo=300, hi=700, seed=5)
+    rng = np.random.default_rng(6)
+    params = np.stack([np.zeros(16), rng.uniform(100, 140, 16), rng.uniform(30, 42, 16), rng.uniform(0.2, 0.4, 16)], 1)
+    g, o = _run_pair(mesh, stim, 2000, scheme="CN", gpu_kw=dict(kernels="fused", thomas="serial"), params=params)
+    np.testing.assert_array_equal(g.vm(), o.vm())
+
+
+def _firing_setup(batch, mesh, width):
+    n = mesh["n_nodes"]
+    F = n.size
+    off = np.concatenate([[0], np.cumsum(n)[:-1]])
+    nmj = n // 2
+    sb, se = vc_lane_stim_range(off, nmj, n, width)
+    fe = firing_table(400, 10, 0.5, 0.1, seed=2)
+    mu = (np.arange(F) * 7) % 10 + 1
+    jit = [[0.1, -0.1, 0.05][: (f % 4)] for f in range(F)]
+    batch.set_firing_table_stimulation(sb, se, fe, mu, call_frequency=0.1, repeat_after_first_call=0.1,
+                                       call_enable_begin=0.0, jitter_lists=jit, log_capacity=4096)
+
+
+@pytest.mark.parametrize("kernels", ["split", "fused"])
+def test_firing_table_stimulation(kernels):
+    mesh, _ = uneven(12, lo=200, hi=600, seed=9)
+    g = _gpu_batch(mesh, scheme="IE", kernels=kernels, thomas="serial")
+    o = OracleBatch(**mesh, scheme="IE")
+    for b in (g, o):
+        _firing_setup(b, mesh, width=4)
+        b.enable_spike_detection(0.0)
+        b.advance(0.0, 2e-3, 1500)
+    np.testing.assert_array_equal(g.vm(), o.vm())
+    np.testing.assert_array_equal(g.spikes()[1], o.spikes()[1])
+    gt, gf = g.stimulation_log()
+    ot, of = o.stimulation_log()
+    assert len(ot) > 0
+    ga = sorted(zip(gf.tolist(), gt.tolist()))
+    oa = sorted(zip(of.tolist(), ot.tolist()))
+    assert ga == oa
+
+
+def test_mixed_precision_error_vs_fp64():
+    mesh, stim = uneven(32, seed=21)
+    g, o = _run_pair(mesh, stim, 4000, scheme="CN", gpu_kw=dict(kernels="fused", precision="mixed"))
+    vg, vo = g.vm(), o.vm()
+    abs_err = float(np.max(np.abs(vg - vo)))
+    gf, of = g.spikes()[0], o.spikes()[0]
+    fired = of > 0
+    assert np.array_equal(gf > 0, fired)
+    shift = int(np.max(np.abs(gf[fired] - of[fired]))) if fired.any() else 0
+    assert shift <= 1, f"spike-time error {shift} steps (Vm max abs err {abs_err:.3e} mV)"
+
+
+def test_schedule_matches_oracle():
+    from opendihu_amd import _lib as L
+    from oracle_binding import oracle_library, schedule
+    lib = L.load_library()
+    for args in [(10.0, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3), (5.0, 2.5e-5, 2.5e-5, 2.5e-5, 2.5e-5, 2.5e-5, 2.5e-5, 2.5e-5),
+                 (100.0, 2e-3, 2e-3, 2e-3, 2e-4, 2e-4, 2e-3, 2e-3)]:
+        a = L.schedule(lib, *args)
+        b = schedule(oracle_library(), *args)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_schedule_driven_run_bit_exact():
+    from opendihu_amd import _lib as L
+    mesh, stim = c1(200, 2.0)
+    lib = L.load_library()
+    sched = L.schedule(lib, 1.0, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3)
+    steps = sched[:, [0, 1, 3, 5]]
+    assert np.all(sched[:, 2] == 1) and np.all(sched[:, 4] == 1)
+    g = _gpu_batch(mesh, scheme="CN", thomas="serial", kernels="fused")
+    o = OracleBatch(**mesh, scheme="CN")
+    for b in (g, o):
+        b.set_current_stimulation(**stim)
+        b.advance_schedule(steps)
+    np.testing.assert_array_equal(g.vm(), o.vm())
+
+
+def test_reference_abi_matches_native():
+    """initializeArrays/computeMonodomain (the reference's dlopen boundary) against the oracle."""
+    import ctypes
+    from opendihu_amd import _lib as L
+    lib = L.load_library()
+    F, n = 3, 301
+    d = L.FmsReferenceDims(n_fibers_to_compute=F, n_instances_per_fiber=n, n_parameters_per_instance=1,
+                           firing_events_rows=2, firing_events_cols=100, frequency_jitter_cols=2,
+                           n_algebraics_for_transfer=1, n_states_for_transfer=2, scheme=0, precision=0, exp_mode=0)
+    L.check(lib, lib.fms_reference_set_dims(ctypes.byref(d)), "set dims")
+    y0 = np.array([-75, 0.05, 0.6, 0.325])
+    alg_idx = np.array([0], np.int32)
+    st_idx = np.array([0, 3], np.int32)
+    fe = np.ones(200, np.uint8)
+    jitter = np.array([0.1, -0.1] * F)
+    mu = np.array([1, 2, 3], np.int32)
+    nmj = np.array([n // 2] * F, np.int32)
+    freq = np.full(F, 0.1)
+    rep = np.full(F, 0.1)
+    beg = np.array([0.0, 1.0, 2.0])
+    last = beg - 1e-13 - 1.0 / freq
+    dp, ip = L._dp, L._ip
+    lib.initializeArrays(y0.ctypes.data_as(dp), alg_idx.ctypes.data_as(ip), st_idx.ctypes.data_as(ip),
+                         fe.tobytes(), jitter.ctypes.data_as(dp), mu.ctypes.data_as(ip), nmj.ctypes.data_as(ip),
+                         last.ctypes.data_as(dp), freq.ctypes.data_as(dp), rep.ctypes.data_as(dp),
+                         beg.ctypes.data_as(dp))
+    params = np.zeros(F * n, np.float32)
+    lengths = np.full(F * (n - 1), 0.01, np.float32)
+    states_out = np.zeros(2 * F * n)
+    alg_out = np.zeros(F * n)
+    fp = ctypes.POINTER(ctypes.c_float)
+    for k in range(1000):
+        lib.computeMonodomain(params.ctypes.data_as(fp), alg_out.ctypes.data_as(dp), states_out.ctypes.data_as(dp),
+                              lengths.ctypes.data_as(fp), k * 2e-3, 2e-3, 1, 1e-3, 1, 2e-3, 1, 0.007656, 20.0)
+    assert L.last_error(lib) == ""
+    o = OracleBatch(n_nodes=[n] * F, element_lengths=lengths.astype(np.float64), prefactor=0.007656, scheme="IE")
+    o.set_firing_table_stimulation(nmj - 1, nmj + 2, fe.reshape(2, 100), mu, freq, rep, beg,
+                                   jitter_lists=[[0.1, -0.1]] * F, last_check_time=last)
+    for k in range(1000):
+        o.advance(k * 2e-3, 2e-3, 1, 1e-3, 1, 2e-3, 1)
+    err = _rel_err(states_out[: F * n], o.vm())
+    assert err <= REL_TOL, err
+    assert _rel_err(states_out[F * n:], o.get_state(3)) <= REL_TOL
+    assert np.max(o.vm()) > 0.0 or np.max(states_out[: F * n]) > -80
