@@ -136,6 +136,12 @@ int fms_schedule(double rc_end, double rc_width, double strang_end, double stran
 
 int fms_last_error(char *buf, int len);
 
+/* timing on the solver's stream: events in slots 0..7, elapsed time between
+   two recorded slots, and the number of kernel launches issued so far */
+int fms_record_event(FmsSolver *solver, int slot);
+int fms_event_elapsed_ms(FmsSolver *solver, int from_slot, int to_slot, double *ms);
+int fms_launch_counts(FmsSolver *solver, long long *fused, long long *reaction, long long *thomas);
+
 /* ---- reference plugin boundary ---- */
 typedef struct {
   int n_fibers_to_compute;

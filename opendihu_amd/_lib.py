@@ -87,7 +87,8 @@ EXPORTED_SYMBOLS = [
     "fms_advance", "fms_advance_schedule", "fms_synchronize", "fms_get_states", "fms_set_states",
     "fms_get_algebraics", "fms_set_algebraics_for_transfer", "fms_total_nodes",
     "fms_enable_spike_detection", "fms_get_spikes", "fms_get_stimulation_log", "fms_schedule",
-    "fms_last_error", "fms_reference_set_dims", "initializeArrays", "computeMonodomain",
+    "fms_last_error", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
+    "fms_reference_set_dims", "initializeArrays", "computeMonodomain",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -125,6 +126,10 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_get_stimulation_log.argtypes = [v, _ip, _dp, _ip, i]
     lib.fms_schedule.argtypes = [d, d, d, d, d, d, d, d, i, _dp]
     lib.fms_last_error.argtypes = [ctypes.c_char_p, i]
+    lib.fms_record_event.argtypes = [v, i]
+    lib.fms_event_elapsed_ms.argtypes = [v, i, i, _dp]
+    lib.fms_launch_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                      ctypes.POINTER(ctypes.c_longlong)]
     lib.fms_reference_set_dims.argtypes = [ctypes.POINTER(FmsReferenceDims)]
     lib.initializeArrays.argtypes = [_dp, _ip, _ip, ctypes.c_char_p, _dp, _ip, _ip, _dp, _dp, _dp, _dp]
     lib.initializeArrays.restype = None

@@ -70,7 +70,7 @@ def build_hh() -> str:
 
 
 def build_model_library(model_c_file: str, parameters_used_as_algebraic=(), parameters_used_as_constant=(),
-                        name: str | None = None) -> str:
+                        name: str | None = None, lib_name: str | None = None) -> str:
     """Generate HIP code for an OpenCOR C model file and compile its library."""
     from .cellml import generate_hip_model
 
@@ -86,12 +86,34 @@ def build_model_library(model_c_file: str, parameters_used_as_algebraic=(), para
                               struct_name=type_name.split("::")[1])
     with open(os.path.join(GEN_DIR, header_name), "w") as f:
         f.write(code)
-    return compile_library(f"libfms_{base}.so", header_name, type_name, extra_include=GEN_DIR)
+    return compile_library(lib_name or f"libfms_{base}.so", header_name, type_name, extra_include=GEN_DIR)
+
+
+# CellML models shipped with the reference's tests (testing/unit_testing/input),
+# with the parameter mappings the reference's settings use for them:
+#   Shorten 2007: parametersUsedAsAlgebraic=[32] (I_Stim), parametersUsedAsConstant=[65] (l_hs)
+#     (testing/unit_testing/src/1_rank/cellml.cpp:193-195)
+#   Hodgkin-Huxley: parametersUsedAsConstant=[2,3,4,5] (i_Stim, g_Na, g_K, g_L); the
+#     generated library cross-checks the generator against the hand-written model
+REFERENCE_MODELS = [
+    ("testing/unit_testing/input/shorten_ocallaghan_davidson_soboleva_2007.c",
+     "shorten_ocallaghan_davidson_soboleva_2007", [32], [65], None),
+    ("testing/unit_testing/input/hodgkin_huxley_1952.c", "hodgkin_huxley_1952", [], [2, 3, 4, 5],
+     "libfms_gen_hodgkin_huxley_1952.so"),
+]
+SHORTEN_LIBRARY = "libfms_shorten_ocallaghan_davidson_soboleva_2007.so"
+GEN_HH_LIBRARY = "libfms_gen_hodgkin_huxley_1952.so"
 
 
 def build_reference_models(reference_root: str) -> list[str]:
-    """Libraries for the CellML models shipped in the reference's test inputs."""
-    return []
+    """Libraries for the CellML models shipped in the reference's test inputs
+    (compiled concurrently, one hipcc process per model)."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = [(os.path.join(reference_root, rel), pa, pc, name, lib)
+            for rel, name, pa, pc, lib in REFERENCE_MODELS if os.path.exists(os.path.join(reference_root, rel))]
+    with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+        futs = [ex.submit(build_model_library, path, pa, pc, name, lib) for path, pa, pc, name, lib in jobs]
+        return [f.result() for f in futs]
 
 
 def main(argv=None) -> int:

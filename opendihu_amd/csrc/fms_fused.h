@@ -41,7 +41,8 @@ __device__ __forceinline__ void fused_node_half(double &V, G *g, G *p, G pbase, 
   for (int j = 0; j < n0D; j++) {
     const int on = flags[j];
     const bool clamp = st.mode == 1 && on && inset;
-    if (st.mode == 2 && inset && M::NP > 0) p[st.stim_param] = on ? (G)st.amplitude[0] : pbase;
+    // p is shared by all nodes a thread owns: set the stimulus parameter for every node
+    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? (G)st.amplitude[0] : pbase;
     heun_node<M, G, E>(V, g, p, t0 + j * dt, dt, clamp, vstim, (alg && j == n0D - 1) ? alg : nullptr);
   }
 }

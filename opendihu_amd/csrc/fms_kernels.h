@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   for (int j = 0; j < n0D; j++) {
     const int on = flags[(long long)f * n0D + j];
     const bool clamp = st.mode == 1 && on && inset;
-    if (st.mode == 2 && inset && M::NP > 0) p[st.stim_param] = on ? (G)st.amplitude[f] : pbase;
+    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? (G)st.amplitude[f] : pbase;
     G alg[M::NA];
     const bool store = alg_transfer != nullptr && j == n0D - 1;
     heun_node<M, G, E>(V, g, p, t0 + j * dt, dt, clamp, vstim, store ? alg : nullptr);

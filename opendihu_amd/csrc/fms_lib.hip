@@ -104,6 +104,10 @@ struct FmsSolver {
   int *d_sfirst = nullptr, *d_scount = nullptr;
   long long step_counter = 0;
 
+  // timing: events on the solver's stream and launch counters
+  hipEvent_t events[8] = {};
+  long long n_launch_fused = 0, n_launch_reaction = 0, n_launch_thomas = 0;
+
   // algebraics for transfer
   int *d_alg_idx = nullptr;
   int n_alg_idx = 0;
@@ -165,6 +169,8 @@ void free_all(FmsSolver *s) {
                   s->d_alg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : s->events)
+    if (e) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -180,6 +186,7 @@ int launch_reaction(FmsSolver *s, double t0, double dt, int n0D, bool store_alg)
                      s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, t0, dt, n0D, s->vstim,
                      store_alg ? s->d_alg : nullptr, s->d_alg_idx, s->n_alg_idx);
   HIP_TRY(hipGetLastError());
+  s->n_launch_reaction++;
   return 0;
 }
 
@@ -203,6 +210,7 @@ int launch_thomas(FmsSolver *s, double dt1) {
                        s->d_vm, dt1, ie);
   }
   HIP_TRY(hipGetLastError());
+  s->n_launch_thomas++;
   return 0;
 }
 
@@ -253,6 +261,7 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D) {
                      spk, tr);
   HIP_TRY(hipGetLastError());
   s->step_counter += K;
+  s->n_launch_fused++;
   return 0;
 }
 
@@ -679,6 +688,31 @@ int fms_get_stimulation_log(FmsSolver *s, int *count, double *times, int *fibers
   const int n = std::min(std::min(*count, s->log_cap), cap);
   if (n > 0 && times) HIP_TRY(hipMemcpy(times, s->d_log_time, n * sizeof(double), hipMemcpyDeviceToHost));
   if (n > 0 && fibers) HIP_TRY(hipMemcpy(fibers, s->d_log_fiber, n * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fms_record_event(FmsSolver *s, int slot) {
+  if (!s || slot < 0 || slot >= 8) return fail(FMS_ERR_ARGUMENT, "event slot must be 0..7");
+  if (!s->events[slot]) HIP_TRY(hipEventCreate(&s->events[slot]));
+  HIP_TRY(hipEventRecord(s->events[slot], s->stream));
+  return 0;
+}
+
+int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
+  if (!s || !ms || from < 0 || from >= 8 || to < 0 || to >= 8 || !s->events[from] || !s->events[to])
+    return fail(FMS_ERR_ARGUMENT, "events not recorded");
+  HIP_TRY(hipEventSynchronize(s->events[to]));
+  float f = 0.f;
+  HIP_TRY(hipEventElapsedTime(&f, s->events[from], s->events[to]));
+  *ms = f;
+  return 0;
+}
+
+int fms_launch_counts(FmsSolver *s, long long *fused, long long *reaction, long long *thomas) {
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  if (fused) *fused = s->n_launch_fused;
+  if (reaction) *reaction = s->n_launch_reaction;
+  if (thomas) *thomas = s->n_launch_thomas;
   return 0;
 }
 

@@ -11,10 +11,15 @@
 // each + - * / below rounds exactly as the reference's g++ -std=c++17 build does.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <math.h>
 
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define FMS_HD __host__ __device__ __forceinline__
+#else
+// host-only compilation of the model headers (CPU checks of generated models)
+#define FMS_HD inline
+#endif
 
 namespace fms {
 

@@ -151,6 +151,20 @@ class FiberBatch:
         self._chk(self.lib.fms_advance_schedule(self.handle, L.as_ptr(steps, L._dp), steps.shape[0], int(n0D)),
                   "fms_advance_schedule")
 
+    def record_event(self, slot: int) -> None:
+        self._chk(self.lib.fms_record_event(self.handle, int(slot)), "fms_record_event")
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = ctypes.c_double()
+        self._chk(self.lib.fms_event_elapsed_ms(self.handle, int(a), int(b), ctypes.byref(ms)), "fms_event_elapsed_ms")
+        return ms.value
+
+    def launch_counts(self) -> dict:
+        f, r, t = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self.lib.fms_launch_counts(self.handle, ctypes.byref(f), ctypes.byref(r), ctypes.byref(t)),
+                  "fms_launch_counts")
+        return dict(fused=f.value, reaction=r.value, thomas=t.value)
+
     def synchronize(self):
         self._chk(self.lib.fms_synchronize(self.handle), "fms_synchronize")
 

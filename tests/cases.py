@@ -54,3 +54,37 @@ def firing_table(n_rows, n_mu, rate_per_ms, dt_row, seed):
     """0/1 table, rows = sampling steps of width dt_row, columns = motor units (Poisson firing)."""
     rng = np.random.default_rng(seed)
     return (rng.random((n_rows, n_mu)) < rate_per_ms * dt_row).astype(np.uint8)
+
+
+def poisson_firing_table(n_rows, n_mu, rate_per_ms, dt_row, duration_ms, seed):
+    """Firing table with one row per sampling step: each motor unit fires at
+    Poisson times (rate_per_ms) and each firing marks duration_ms of rows."""
+    rng = np.random.default_rng(seed)
+    table = np.zeros((n_rows, n_mu), np.uint8)
+    t_end = n_rows * dt_row
+    width = max(1, int(round(duration_ms / dt_row)))
+    for mu in range(n_mu):
+        t = rng.exponential(1.0 / rate_per_ms)
+        while t < t_end:
+            r0 = int(t / dt_row)
+            table[r0:r0 + width, mu] = 1
+            t += rng.exponential(1.0 / rate_per_ms)
+    return table
+
+
+def c2(n_fibers=1024, n_nodes=1481, length_cm=14.8, dt=2.5e-5, end_time=5.0, n_mu=10, rate_hz=20.0,
+       seed_mu=1, seed_fire=2, lane_width=4):
+    """57-state-class Shorten workload: equal fibres, IE, motor units with Poisson
+    firing at the centre node; stimulation follows the reference's firing-table
+    semantics with the table sampled once per split step."""
+    n = np.full(n_fibers, n_nodes, np.int32)
+    h = np.full(n_fibers * (n_nodes - 1), length_cm / (n_nodes - 1))
+    pf = np.full(n_fibers, SIGMA / (AM * CM))
+    mu = np.random.default_rng(seed_mu).integers(1, n_mu + 1, size=n_fibers).astype(np.int32)
+    rows = int(round(end_time / dt))
+    table = poisson_firing_table(rows, n_mu, rate_hz * 1e-3, dt, 0.1, seed_fire)
+    off = np.arange(n_fibers, dtype=np.int64) * n_nodes
+    sb, se = vc_lane_stim_range(off, np.full(n_fibers, n_nodes // 2), n, lane_width)
+    stim = dict(stim_begin=sb, stim_end=se, firing_events=table, motor_unit=mu, call_frequency=1.0 / dt,
+                repeat_after_first_call=0.0, call_enable_begin=0.0)
+    return dict(n_nodes=n, element_lengths=h, prefactor=pf), stim
