@@ -136,6 +136,11 @@ int fms_schedule(double rc_end, double rc_width, double strang_end, double stran
 
 int fms_last_error(char *buf, int len);
 
+/* schedule knobs of the fused kernel (the autotuner's search space) */
+#define FMS_TUNE_STEPS_PER_LAUNCH 1   /* split steps per fused launch (default 512) */
+#define FMS_TUNE_RESIDENT_GATES 2     /* -1 auto, 0 stream gates from HBM, 1 keep them in registers */
+int fms_set_tuning(FmsSolver *solver, int key, int value);
+
 /* timing on the solver's stream: events in slots 0..7, elapsed time between
    two recorded slots, and the number of kernel launches issued so far */
 int fms_record_event(FmsSolver *solver, int slot);

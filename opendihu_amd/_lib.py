@@ -18,6 +18,7 @@ FMS_EXP_APPROXIMATE, FMS_EXP_LIBM = 0, 1
 FMS_THOMAS_SERIAL, FMS_THOMAS_PARTITIONED = 0, 1
 FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
 FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
+FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES = 1, 2
 
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -87,7 +88,7 @@ EXPORTED_SYMBOLS = [
     "fms_advance", "fms_advance_schedule", "fms_synchronize", "fms_get_states", "fms_set_states",
     "fms_get_algebraics", "fms_set_algebraics_for_transfer", "fms_total_nodes",
     "fms_enable_spike_detection", "fms_get_spikes", "fms_get_stimulation_log", "fms_schedule",
-    "fms_last_error", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
+    "fms_last_error", "fms_set_tuning", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain",
 ]
 
@@ -126,6 +127,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_get_stimulation_log.argtypes = [v, _ip, _dp, _ip, i]
     lib.fms_schedule.argtypes = [d, d, d, d, d, d, d, d, i, _dp]
     lib.fms_last_error.argtypes = [ctypes.c_char_p, i]
+    lib.fms_set_tuning.argtypes = [v, i, i]
     lib.fms_record_event.argtypes = [v, i]
     lib.fms_event_elapsed_ms.argtypes = [v, i, i, _dp]
     lib.fms_launch_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),

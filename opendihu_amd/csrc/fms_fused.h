@@ -28,22 +28,16 @@ struct DevSpikes {
   int step0;            // global index of the step before this launch's first step
 };
 
-struct DevTransfer {
-  double *alg;          // [n_idx*N] or nullptr
-  const int *idx;
-  int n_idx;
-};
-
 template <class M, class G, class E>
-__device__ __forceinline__ void fused_node_half(double &V, G *g, G *p, G pbase, const DevStim &st, bool inset,
-                                                const unsigned char *flags, double t0, double dt, int n0D,
-                                                double vstim, G *alg) {
+__device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, double pbase, const DevStim &st,
+                                                bool inset, const unsigned char *flags, double t0, double dt,
+                                                int n0D, double vstim) {
   for (int j = 0; j < n0D; j++) {
     const int on = flags[j];
     const bool clamp = st.mode == 1 && on && inset;
     // p is shared by all nodes a thread owns: set the stimulus parameter for every node
-    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? (G)st.amplitude[0] : pbase;
-    heun_node<M, G, E>(V, g, p, t0 + j * dt, dt, clamp, vstim, (alg && j == n0D - 1) ? alg : nullptr);
+    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[0] : pbase;
+    heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
   }
 }
 
@@ -52,7 +46,7 @@ template <class M, class G, class E, int T, int MMAX, int NPT>
 __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *__restrict__ vm,
                                              G *__restrict__ gates, const double *__restrict__ steps, int K,
                                              int n0D, double vstim, int ie, int serial, const int *__restrict__ order,
-                                             DevSpikes spk, DevTransfer tr) {
+                                             DevSpikes spk) {
   extern __shared__ double sh[];
   const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x;
@@ -67,10 +61,10 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
 
   for (int i = tid; i < n; i += T) lv[i] = vm[off + i];
 
-  G p[M::NP > 0 ? M::NP : 1];
+  double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
-  for (int k = 0; k < M::NP; k++) p[k] = (G)fb.params[(long long)f * M::NP + k];
-  const G pbase = M::NP > 0 ? p[st.stim_param] : G(0);
+  for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
+  const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   // the current-pulse amplitude is read through a per-fibre view
   DevStim stf = st;
@@ -109,8 +103,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         const int i = tid + q * T;
         if (i < n) {
           double V = lv[i];
-          fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim,
-                                   nullptr);
+          fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
           lv[i] = V;
         }
       }
@@ -120,7 +113,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
 #pragma unroll
         for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + off + i];
         double V = lv[i];
-        fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim, nullptr);
+        fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
         lv[i] = V;
 #pragma unroll
         for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = g[k];
@@ -153,21 +146,8 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     }
     __syncthreads();
 
-    const bool store_alg = tr.alg != nullptr && !more;
     auto node = [&](int i, double &V, G *g, double &vp, int &sf, int &sc) {
-      G alg[M::NA];
-      fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim,
-                               store_alg ? alg : nullptr);
-      if (store_alg) {
-        for (int k = 0; k < tr.n_idx; k++) {
-          const int a = tr.idx[k];
-          double v = 0;
-#pragma unroll
-          for (int q = 0; q < M::NA; q++)
-            if (q == a) v = (double)alg[q];
-          tr.alg[(long long)k * N + off + i] = v;
-        }
-      }
+      fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim);
       if (spk.enabled) {
         if (vp < spk.threshold && V >= spk.threshold) {
           sc++;
@@ -177,7 +157,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
       }
       if (more)
         fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D,
-                                 vstim, nullptr);
+                                 vstim);
     };
 
     if (NPT > 0) {

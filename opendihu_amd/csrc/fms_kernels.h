@@ -62,29 +62,34 @@ struct DevStim {
 // ----------------------------------------------------------------------------
 // Heun step of one node, reference form (03_generator_vc.cpp:1017-1130):
 //   y* = y + dt f(y); [clamp y*_0]; y += 0.5 dt (f(y) + f(y*)); [clamp y_0]
-// Vm is FP64 in both precisions; the gates and the RHS arithmetic use G.
+// The arithmetic is FP64 in both precisions; G is the storage type of the
+// gates (double, or float for the mixed form: gates are rounded to FP32 once
+// per Heun step, Vm and the rates stay FP64, so the rate functions'
+// removable singularities such as alpha_m at V = -50 mV are no more likely to
+// be hit than in the FP64 reference).
 // ----------------------------------------------------------------------------
-template <class M, class G, class E>
-FMS_HD void heun_node(double &V, G *g, const G *p, double t, double dt, bool clamp, double vstim,
-                      G *alg_out = nullptr) {
-  G y[M::NS], r[M::NS], ys[M::NS], rs[M::NS], a[M::NA], as[M::NA];
-  y[0] = (G)V;
+// ALG: also return the algebraics of the second evaluation (the reference's
+// algebraicsForTransfer); a compile-time switch so that the 71 algebraics of a
+// large model are not kept live in the hot variant.
+template <class M, class G, class E, bool ALG = false>
+FMS_HD void heun_node(double &V, G *g, const double *p, double t, double dt, bool clamp, double vstim,
+                      double *alg_out = nullptr) {
+  double y[M::NS], r[M::NS], ys[M::NS], rs[M::NS], a[M::NA], as[M::NA];
+  y[0] = V;
 #pragma unroll
-  for (int k = 1; k < M::NS; k++) y[k] = g[k - 1];
-  M::template rhs<G, E>(y, p, t, r, a);
-  double Vs = V + dt * (double)r[0];
-  const G gdt = (G)dt;
+  for (int k = 1; k < M::NS; k++) y[k] = (double)g[k - 1];
+  M::template rhs<double, E>(y, p, t, r, a);
 #pragma unroll
-  for (int k = 1; k < M::NS; k++) ys[k] = y[k] + gdt * r[k];
-  if (clamp) Vs = vstim;
-  ys[0] = (G)Vs;
-  M::template rhs<G, E>(ys, p, t, rs, as);
-  V += 0.5 * dt * ((double)r[0] + (double)rs[0]);
-  const G ghdt = (G)(0.5 * dt);
+  for (int k = 0; k < M::NS; k++) ys[k] = y[k] + dt * r[k];
+  if (clamp) ys[0] = vstim;
+  M::template rhs<double, E>(ys, p, t, rs, as);
 #pragma unroll
-  for (int k = 1; k < M::NS; k++) g[k - 1] = y[k] + ghdt * (r[k] + rs[k]);
-  if (clamp) V = vstim;
-  if (alg_out) {
+  for (int k = 0; k < M::NS; k++) y[k] += 0.5 * dt * (r[k] + rs[k]);
+  if (clamp) y[0] = vstim;
+  V = y[0];
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) g[k - 1] = (G)y[k];
+  if constexpr (ALG) {
 #pragma unroll
     for (int k = 0; k < M::NA; k++) alg_out[k] = as[k];
   }
@@ -163,31 +168,32 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   if (i >= N) return;
   const int f = fb.node_fiber[i];
   const int li = (int)(i - fb.fiber_off[f]);
-  G p[M::NP > 0 ? M::NP : 1];
+  double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
-  for (int k = 0; k < M::NP; k++) p[k] = (G)fb.params[(long long)f * M::NP + k];
+  for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
   const bool inset = li >= fb.stim_begin[f] && li < fb.stim_end[f];
   double V = vm[i];
   G g[M::NS - 1];
 #pragma unroll
   for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
-  const G pbase = M::NP > 0 ? p[st.stim_param] : G(0);
+  const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   for (int j = 0; j < n0D; j++) {
     const int on = flags[(long long)f * n0D + j];
     const bool clamp = st.mode == 1 && on && inset;
-    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? (G)st.amplitude[f] : pbase;
-    G alg[M::NA];
-    const bool store = alg_transfer != nullptr && j == n0D - 1;
-    heun_node<M, G, E>(V, g, p, t0 + j * dt, dt, clamp, vstim, store ? alg : nullptr);
-    if (store) {
+    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[f] : pbase;
+    if (alg_transfer != nullptr && j == n0D - 1) {
+      double alg[M::NA];
+      heun_node<M, G, E, true>(V, g, p, t0 + j * dt, dt, clamp, vstim, alg);
       for (int k = 0; k < n_alg_transfer; k++) {
         const int a = alg_idx[k];
         double v = 0;
 #pragma unroll
         for (int q = 0; q < M::NA; q++)
-          if (q == a) v = (double)alg[q];
+          if (q == a) v = alg[q];
         alg_transfer[(long long)k * N + i] = v;
       }
+    } else {
+      heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
     }
   }
   vm[i] = V;

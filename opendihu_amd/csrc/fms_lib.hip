@@ -104,6 +104,10 @@ struct FmsSolver {
   int *d_sfirst = nullptr, *d_scount = nullptr;
   long long step_counter = 0;
 
+  // schedule knobs (fms_set_tuning): steps per fused launch, gate residency
+  int tune_chunk = 512;
+  int tune_resident = -1;   // -1 auto, 0 stream gates from HBM, 1 keep gates in registers
+
   // timing: events on the solver's stream and launch counters
   hipEvent_t events[8] = {};
   long long n_launch_fused = 0, n_launch_reaction = 0, n_launch_thomas = 0;
@@ -250,15 +254,11 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D) {
   spk.first_step = s->d_sfirst;
   spk.count = s->d_scount;
   spk.step0 = (int)s->step_counter;
-  fms::DevTransfer tr;
-  tr.alg = s->d_alg;
-  tr.idx = s->d_alg_idx;
-  tr.n_idx = s->n_alg_idx;
   const size_t lds = (size_t)(16 * T + s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
   hipLaunchKernelGGL((fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->stream, s->fibers(),
                      s->stim(), s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim,
                      s->scheme == FMS_IMPLICIT_EULER ? 1 : 0, s->thomas == FMS_THOMAS_SERIAL ? 1 : 0, s->d_order,
-                     spk, tr);
+                     spk);
   HIP_TRY(hipGetLastError());
   s->step_counter += K;
   s->n_launch_fused++;
@@ -272,18 +272,40 @@ template <class G, class E>
 int launch_fused(FmsSolver *s, const double *d_steps, int K, int n0D) {
   constexpr int T = FUSED_T;
   const int npt = (s->nmax + T - 1) / T;
-  const bool small_model = Model::NS <= 8;
-  if (small_model && npt <= 4) return launch_fused_npt<G, E, 4, 8>(s, d_steps, K, n0D);
-  if (small_model && npt <= 8) return launch_fused_npt<G, E, 8, 8>(s, d_steps, K, n0D);
+  if constexpr (Model::NS <= 8) {
+    const bool resident = s->tune_resident != 0;
+    if (resident && npt <= 4) return launch_fused_npt<G, E, 4, 8>(s, d_steps, K, n0D);
+    if (resident && npt <= 8) return launch_fused_npt<G, E, 8, 8>(s, d_steps, K, n0D);
+  }
   if (npt <= 8) return launch_fused_npt<G, E, 0, 8>(s, d_steps, K, n0D);
   return launch_fused_npt<G, E, 0, 16>(s, d_steps, K, n0D);
 }
 
 template <class G, class E>
+int run(FmsSolver *s, const double *h_steps, int K, int n0D, bool fused_only);
+
+template <class G, class E>
 int run(FmsSolver *s, const double *h_steps, int K, int n0D) {
   if (s->kernels == FMS_KERNELS_SPLIT) return run_split<G, E>(s, h_steps, K, n0D);
+  // the fused kernel does not return algebraics: with algebraicsForTransfer
+  // requested, the last step of the call runs on the split kernels (same
+  // arithmetic, bit-identical states)
+  if (s->d_alg) {
+    int rc = 0;
+    if (K > 1 && (rc = run<G, E>(s, h_steps, K - 1, n0D, true))) return rc;
+    if (s->thomas == FMS_THOMAS_SERIAL && !s->d_cA) {
+      if (dalloc(&s->d_cA, (size_t)s->N) || dalloc(&s->d_dA, (size_t)s->N)) return FMS_ERR_HIP;
+    }
+    return run_split<G, E>(s, h_steps + 4 * (size_t)(K - 1), 1, n0D);
+  }
+  return run<G, E>(s, h_steps, K, n0D, true);
+}
+
+template <class G, class E>
+int run(FmsSolver *s, const double *h_steps, int K, int n0D, bool fused_only) {
+  (void)fused_only;
   // fused: upload the step table (4 doubles + one look-ahead record) and launch in chunks
-  const int chunk = 512;
+  const int chunk = s->tune_chunk;
   if (s->steps_cap < 4 * (K + 1)) {
     if (s->d_steps) HIP_TRY(hipFree(s->d_steps));
     s->steps_cap = 4 * (K + 1);
@@ -689,6 +711,24 @@ int fms_get_stimulation_log(FmsSolver *s, int *count, double *times, int *fibers
   if (n > 0 && times) HIP_TRY(hipMemcpy(times, s->d_log_time, n * sizeof(double), hipMemcpyDeviceToHost));
   if (n > 0 && fibers) HIP_TRY(hipMemcpy(fibers, s->d_log_fiber, n * sizeof(int), hipMemcpyDeviceToHost));
   return 0;
+}
+
+int fms_set_tuning(FmsSolver *s, int key, int value) {
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  switch (key) {
+    case FMS_TUNE_STEPS_PER_LAUNCH:
+      if (value < 1) return fail(FMS_ERR_ARGUMENT, "steps per launch must be >= 1");
+      s->tune_chunk = value;
+      return 0;
+    case FMS_TUNE_RESIDENT_GATES:
+      if (value < -1 || value > 1) return fail(FMS_ERR_ARGUMENT, "resident must be -1, 0 or 1");
+      if (value == 1 && ((s->nmax + FUSED_T - 1) / FUSED_T > 8 || Model::NS > 8))
+        return fail(FMS_ERR_UNSUPPORTED, "resident gates need fibres of <= 2048 nodes and <= 8 states");
+      s->tune_resident = value;
+      return 0;
+    default:
+      return fail(FMS_ERR_ARGUMENT, "unknown tuning key");
+  }
 }
 
 int fms_record_event(FmsSolver *s, int slot) {

@@ -151,6 +151,15 @@ class FiberBatch:
         self._chk(self.lib.fms_advance_schedule(self.handle, L.as_ptr(steps, L._dp), steps.shape[0], int(n0D)),
                   "fms_advance_schedule")
 
+    def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None) -> None:
+        """fused-kernel schedule: split steps per launch; gates in registers (1), streamed (0) or auto (-1)"""
+        if steps_per_launch is not None:
+            self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_STEPS_PER_LAUNCH, int(steps_per_launch)),
+                      "fms_set_tuning")
+        if resident_gates is not None:
+            self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_RESIDENT_GATES, int(resident_gates)),
+                      "fms_set_tuning")
+
     def record_event(self, slot: int) -> None:
         self._chk(self.lib.fms_record_event(self.handle, int(slot)), "fms_record_event")
 

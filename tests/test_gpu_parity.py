@@ -226,3 +226,33 @@ def test_reference_abi_matches_native():
     assert err <= REL_TOL, err
     assert _rel_err(states_out[F * n:], o.get_state(3)) <= REL_TOL
     assert np.max(o.vm()) > 0.0 or np.max(states_out[: F * n]) > -80
+
+
+@pytest.mark.parametrize("resident", [0, 1])
+def test_fused_gate_residency_bit_exact(resident):
+    """gates kept in registers or streamed from HBM by the fused kernel give the
+    same bits as the split kernels with the serial solve"""
+    mesh, stim = uneven(10, lo=300, hi=1500, seed=8)
+    runs = []
+    for kernels in ("split", "fused"):
+        g = _gpu_batch(mesh, scheme="CN", thomas="serial", kernels=kernels)
+        if kernels == "fused":
+            g.set_tuning(steps_per_launch=97, resident_gates=resident)
+        g.set_current_stimulation(**stim)
+        g.advance(0.0, 1e-3, 600)
+        runs.append(g.get_states())
+    np.testing.assert_array_equal(runs[0], runs[1])
+
+
+def test_algebraics_for_transfer_fused_equals_split():
+    mesh, stim = uneven(6, lo=300, hi=600, seed=12)
+    out = []
+    for kernels in ("split", "fused"):
+        g = _gpu_batch(mesh, scheme="IE", thomas="serial", kernels=kernels)
+        g.set_current_stimulation(**stim)
+        g.set_algebraics_for_transfer([0, 4, 8])
+        g.advance(0.0, 1e-3, 700)
+        out.append((g.get_states(), g.get_algebraics()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.all(np.isfinite(out[0][1])) and np.any(out[0][1] != 0)
