@@ -10,9 +10,12 @@ resident in HBM when the timed region starts.
 
 The JSON line carries:
   value            FP64 fiber-node-steps/s over the timed steps
-  roofline         dominant kernel (k_fused) vs HBM: algorithmic bytes per launch
-                   (B_alg per node-step x nodes x steps in the launch) / mean
-                   launch time from HIP events on the solver's stream
+  roofline         dominant kernel (k_fused) against the FP64 vector peak (the
+                   kernel is VALU-bound): algorithmic ops per launch (F_alg per
+                   node-step x nodes x steps in the launch; F_alg from the model's
+                   op count, each + - * / exp log = 1) / mean launch time from HIP
+                   events on the solver's stream; "hbm" holds the same launch
+                   against the HBM roof with B_alg bytes per node-step
   cpu_baseline     the FP64 CPU oracle (the reference's vc-path restatement) on the
                    host cores, over a bounded sample of the same workload
   mixed            the FP32-state / FP64-Vm form on the same workload
@@ -36,6 +39,10 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
+
+# HBM bytes per fiber-node-step of k_fused measured with rocprofv3 (FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000, factors from a 1 GiB copy), profiles/k_fused_shorten56_1024x1481.md
+MEASURED_TRAFFIC_PER_NODE_STEP = {("c2", "fused"): 941.9}
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
@@ -157,13 +164,16 @@ def main(argv=None):
     wall, ev_ms, launches = time_gpu(g, dt, args.steps, args.warmup)
     n_total = g.n_total
     value = n_total * args.steps / wall
+    from opendihu_amd import _lib as L
     b_alg = bytes_per_node_step(g.n_states, "fp64")
+    f_alg = L.flops_per_node_step(g.lib, g.n_states, 1, scheme)
     kernel = "k_fused" if args.kernels == "fused" else "k_reaction"
     n_launch = launches["fused"] if args.kernels == "fused" else launches["reaction"]
     mean_launch_ms = ev_ms / max(1, n_launch)
     steps_per_launch = args.steps / max(1, n_launch)
-    bytes_per_launch = b_alg * n_total * (steps_per_launch if args.kernels == "fused" else 0.5)
-    achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    node_steps_per_launch = n_total * (steps_per_launch if args.kernels == "fused" else 0.5)
+    achieved_gbs = b_alg * node_steps_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    achieved_tflops = f_alg * node_steps_per_launch / (mean_launch_ms * 1e-3) / 1e12
     g.close()
 
     mixed = None
@@ -193,16 +203,19 @@ def main(argv=None):
         "data": "synthetic (seeded fibre meshes, motor units and firing times; no checkpoints)",
         "config": dict(desc, kernels=args.kernels, thomas="partitioned", exp="approximate (1+x/1024)^1024"),
         "roofline": {
-            "bound": "hbm",
+            "bound": "valu",
             "kernel": kernel,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_node_step": b_alg,
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": (MEASURED_TRAFFIC_PER_NODE_STEP[(args.workload, args.kernels)] * node_steps_per_launch
+                        if (args.workload, args.kernels) in MEASURED_TRAFFIC_PER_NODE_STEP else None),
+            "flops_per_node_step": f_alg,
             "launches": n_launch,
             "mean_launch_ms": mean_launch_ms,
+            "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_node_step": b_alg},
         },
         "cpu_baseline": cpu,
         "mixed": mixed,

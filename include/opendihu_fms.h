@@ -98,6 +98,9 @@ typedef struct {
 /* model compiled into this library */
 int fms_model_info(int *n_states, int *n_algebraics, int *n_params, char *name, int name_len);
 int fms_model_defaults(double *states /* [n_states] */, double *params /* [n_params] */);
+/* algorithmic operations of one right-hand-side evaluation (each + - * / and
+   each exp/log counts one; integer powers count their multiplications) */
+int fms_model_ops(double *ops_rhs, double *divs_rhs, double *exps_rhs);
 
 int fms_create(const FmsConfig *config, FmsSolver **solver);
 int fms_destroy(FmsSolver *solver);

@@ -84,7 +84,7 @@ class FmsReferenceDims(ctypes.Structure):
 
 # every symbol declared in include/opendihu_fms.h
 EXPORTED_SYMBOLS = [
-    "fms_model_info", "fms_model_defaults", "fms_create", "fms_destroy", "fms_set_stimulation",
+    "fms_model_info", "fms_model_defaults", "fms_model_ops", "fms_create", "fms_destroy", "fms_set_stimulation",
     "fms_advance", "fms_advance_schedule", "fms_synchronize", "fms_get_states", "fms_set_states",
     "fms_get_algebraics", "fms_set_algebraics_for_transfer", "fms_total_nodes",
     "fms_enable_spike_detection", "fms_get_spikes", "fms_get_stimulation_log", "fms_schedule",
@@ -111,6 +111,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     d, i, v = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
     lib.fms_model_info.argtypes = [_ip, _ip, _ip, ctypes.c_char_p, i]
     lib.fms_model_defaults.argtypes = [_dp, _dp]
+    lib.fms_model_ops.argtypes = [_dp, _dp, _dp]
     lib.fms_create.argtypes = [ctypes.POINTER(FmsConfig), ctypes.POINTER(v)]
     lib.fms_destroy.argtypes = [v]
     lib.fms_set_stimulation.argtypes = [v, ctypes.POINTER(FmsStimulation)]
@@ -165,6 +166,24 @@ def model_info(lib: ctypes.CDLL) -> tuple[int, int, int, str]:
     buf = ctypes.create_string_buffer(256)
     check(lib, lib.fms_model_info(ctypes.byref(ns), ctypes.byref(na), ctypes.byref(nparam), buf, 256), "fms_model_info")
     return ns.value, na.value, nparam.value, buf.value.decode()
+
+
+def model_ops(lib: ctypes.CDLL) -> dict:
+    o, d, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib, lib.fms_model_ops(ctypes.byref(o), ctypes.byref(d), ctypes.byref(e)), "fms_model_ops")
+    return dict(ops_rhs=o.value, divs_rhs=d.value, exps_rhs=e.value)
+
+
+# algorithmic ops per node of one diffusion solve, as the reference's row
+# assembly and Thomas recurrence are written (fast_monodomain_solver_compute.tpp:336-486)
+THOMAS_OPS = {"IE": 60, "CN": 68}
+
+
+def flops_per_node_step(lib: ctypes.CDLL, n_states: int, n0D: int = 1, scheme: str = "IE") -> float:
+    """algorithmic ops per fiber-node-step: two reaction halves of n0D Heun steps
+    (two rhs evaluations + 5 ops per state each) plus one diffusion solve"""
+    ops = model_ops(lib)["ops_rhs"]
+    return 2 * n0D * (2 * ops + 5 * n_states) + THOMAS_OPS[scheme]
 
 
 def schedule(lib: ctypes.CDLL, rc_end, rc_width, strang_end, strang_width, heun_end, heun_width, ie_end,

@@ -148,6 +148,60 @@ class _Translator:
         return code, k + 1
 
 
+def count_ops(model: OpenCorModel, parameters_used_as_algebraic=()) -> dict:
+    """Algorithmic operation count of one right-hand-side evaluation: each binary
+    + - * / is one op, each exp/log call one op, pow(x,n) with integer n its
+    n-1 multiplications (a reciprocal power adds one division)."""
+    p_alg = set(parameters_used_as_algebraic)
+    c = dict(add=0, mul=0, div=0, exp=0, log=0, cmp=0)
+    for kind, k, expr in model.lines:
+        if kind == "ALGEBRAIC" and k in p_alg:
+            continue
+        toks = _tokenize(expr)
+        prev = None
+        i = 0
+        while i < len(toks):
+            t = toks[i]
+            if t in ("+", "-"):
+                if prev is not None and (prev in (")", "]") or re.match(r"^[\w.]", prev)):
+                    c["add"] += 1
+            elif t == "*":
+                c["mul"] += 1
+            elif t == "/":
+                c["div"] += 1
+            elif t == "exp":
+                c["exp"] += 1
+            elif t == "log":
+                c["log"] += 1
+            elif t in ("<", ">", "<=", ">=", "==", "!="):
+                c["cmp"] += 1
+            elif t == "pow":
+                # pow ( base , n ): count by exponent; base tokens are visited as usual
+                depth, j = 0, i + 1
+                while j < len(toks):
+                    if toks[j] == "(":
+                        depth += 1
+                    elif toks[j] == ")":
+                        depth -= 1
+                        if depth == 0:
+                            break
+                    elif toks[j] == "," and depth == 1:
+                        comma = j
+                    j += 1
+                e = "".join(toks[comma + 1:j])
+                if re.fullmatch(r"[-+]?\d+(\.0*)?", e):
+                    n = int(float(e))
+                    c["mul"] += max(abs(n) - 1, 0)
+                    c["div"] += 1 if n < 0 else 0
+                    toks = toks[:comma] + toks[j:]
+                else:
+                    c["exp"] += 1
+            prev = t
+            i += 1
+    c["total"] = c["add"] + c["mul"] + c["div"] + c["exp"] + c["log"]
+    return c
+
+
 def _cxx_double(v: float) -> str:
     r = repr(float(v))
     return r if ("." in r or "e" in r or "inf" in r or "nan" in r) else r + ".0"
@@ -183,6 +237,7 @@ def generate_hip_model(text: str, name: str, parameters_used_as_algebraic=(), pa
     needed = set(used_consts)
     const_code = [c for (k, _), c in zip(model.constant_lines, const_code) if k in needed]
 
+    ops = count_ops(model, p_alg)
     y0 = ", ".join(_cxx_double(v) for v in model.state_initial)
     p0s = ", ".join(_cxx_double(v) for v in p0) if p0 else "0.0"
     guard = re.sub(r"[^A-Za-z0-9_]", "_", struct).upper()
@@ -198,6 +253,10 @@ struct {struct} {{
   static constexpr int NA = {model.n_algebraics};
   static constexpr int NP = {n_params};
   static constexpr int STIM_PARAM = 0;
+  // algorithmic ops of one rhs evaluation: {ops}
+  static constexpr double OPS_RHS = {ops["total"]};
+  static constexpr double DIVS_RHS = {ops["div"]};
+  static constexpr double EXPS_RHS = {ops["exp"] + ops["log"]};
   static constexpr const char *name() {{ return "{name}"; }}
   static FMS_HD double y0(int k) {{
     const double v[NS] = {{{y0}}};

@@ -365,6 +365,13 @@ int fms_model_info(int *n_states, int *n_algebraics, int *n_params, char *name, 
   return 0;
 }
 
+int fms_model_ops(double *ops_rhs, double *divs_rhs, double *exps_rhs) {
+  if (ops_rhs) *ops_rhs = Model::OPS_RHS;
+  if (divs_rhs) *divs_rhs = Model::DIVS_RHS;
+  if (exps_rhs) *exps_rhs = Model::EXPS_RHS;
+  return 0;
+}
+
 int fms_model_defaults(double *states, double *params) {
   if (states)
     for (int k = 0; k < Model::NS; k++) states[k] = Model::y0(k);

@@ -20,6 +20,11 @@ struct ModelHH {
   static constexpr int NA = 9;   // algebraics of the model file
   static constexpr int NP = 4;   // i_Stim, g_Na, g_K, g_L
   static constexpr int STIM_PARAM = 0;
+  // algorithmic ops of one rhs evaluation, counted as the code generator counts
+  // them for the model file: 23 add/sub, 22 mul, 10 div, 6 exp
+  static constexpr double OPS_RHS = 61;
+  static constexpr double DIVS_RHS = 10;
+  static constexpr double EXPS_RHS = 6;
   static constexpr const char *name() { return "hodgkin_huxley_1952"; }
   static FMS_HD double y0(int k) {
     const double v[NS] = {-75, 0.05, 0.6, 0.325};
