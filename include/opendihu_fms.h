@@ -142,6 +142,9 @@ int fms_last_error(char *buf, int len);
 /* schedule knobs of the fused kernel (the autotuner's search space) */
 #define FMS_TUNE_STEPS_PER_LAUNCH 1   /* split steps per fused launch (default 512) */
 #define FMS_TUNE_RESIDENT_GATES 2     /* -1 auto, 0 stream gates from HBM, 1 keep them in registers */
+/* onlyComputeIfHasBeenStimulated (0/1, default 0): a fibre's reaction runs from
+   the Heun step of its first stimulation on; diffusion always runs */
+#define FMS_TUNE_ONLY_IF_STIMULATED 3
 int fms_set_tuning(FmsSolver *solver, int key, int value);
 
 /* timing on the solver's stream: events in slots 0..7, elapsed time between

@@ -18,7 +18,7 @@ FMS_EXP_APPROXIMATE, FMS_EXP_LIBM = 0, 1
 FMS_THOMAS_SERIAL, FMS_THOMAS_PARTITIONED = 0, 1
 FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
 FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
-FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES = 1, 2
+FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES, FMS_TUNE_ONLY_IF_STIMULATED = 1, 2, 3
 
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -33,7 +33,8 @@ __device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, doub
                                                 bool inset, const unsigned char *flags, double t0, double dt,
                                                 int n0D, double vstim) {
   for (int j = 0; j < n0D; j++) {
-    const int on = flags[j];
+    if (!(flags[j] & FLAG_COMPUTE)) continue;
+    const int on = flags[j] & FLAG_STIM;
     const bool clamp = st.mode == 1 && on && inset;
     // p is shared by all nodes a thread owns: set the stimulus parameter for every node
     if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[0] : pbase;
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
 
   // flags of the first half step of step 0
   if (tid == 0)
-    for (int j = 0; j < n0D; j++) sflags[j] = (unsigned char)stim_flag(st, f, steps[0] + j * steps[2]);
+    for (int j = 0; j < n0D; j++) sflags[j] = stim_flag(st, f, steps[0] + j * steps[2]);
   __syncthreads();
 
   // pass 0: first reaction half of step 0
@@ -138,10 +139,10 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
 
     // flags: second half of step s, then first half of step s+1
     if (tid == 0) {
-      for (int j = 0; j < n0D; j++) sflags[j] = (unsigned char)stim_flag(st, f, t_mid + j * dt0);
+      for (int j = 0; j < n0D; j++) sflags[j] = stim_flag(st, f, t_mid + j * dt0);
       if (more) {
         const double t1 = st_s[4], dt0n = st_s[6];
-        for (int j = 0; j < n0D; j++) sflags[n0D + j] = (unsigned char)stim_flag(st, f, t1 + j * dt0n);
+        for (int j = 0; j < n0D; j++) sflags[n0D + j] = stim_flag(st, f, t1 + j * dt0n);
       }
     }
     __syncthreads();

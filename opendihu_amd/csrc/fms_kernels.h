@@ -57,6 +57,9 @@ struct DevStim {
   // current pulse (extension)
   const double *onset, *duration, *amplitude;
   int stim_param;
+  // onlyComputeIfHasBeenStimulated: a fibre's reaction runs from the Heun step
+  // of its first stimulation on (fast_monodomain_solver_compute.tpp:201-205)
+  int only_if_stimulated;
 };
 
 // ----------------------------------------------------------------------------
@@ -141,17 +144,26 @@ __device__ __forceinline__ int current_on(const DevStim &s, int f, double t) {
   return t >= s.onset[f] - 1e-13 && t < s.onset[f] + s.duration[f] - 1e-13;
 }
 
-__device__ __forceinline__ int stim_flag(const DevStim &s, int f, double t) {
-  if (s.mode == 1) return stim_check(s, f, t);
-  if (s.mode == 2) return current_on(s, f, t);
-  return 0;
+// per Heun step flag byte: bit 0 = stimulate, bit 1 = compute the reaction
+constexpr unsigned char FLAG_STIM = 1, FLAG_COMPUTE = 2;
+
+__device__ __forceinline__ unsigned char stim_flag(const DevStim &s, int f, double t) {
+  int stim = 0;
+  if (s.mode == 1) {
+    stim = stim_check(s, f, t);
+  } else if (s.mode == 2) {
+    stim = current_on(s, f, t);
+    if (stim) s.has_been[f] = 1;
+  }
+  const bool compute = !s.only_if_stimulated || s.has_been[f];
+  return (unsigned char)((stim ? FLAG_STIM : 0) | (compute ? FLAG_COMPUTE : 0));
 }
 
 // one thread per fibre: flags[f*n0D + j] for the n0D Heun steps of a half step
 __global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, double t0, double dt, int n0D) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n_fibers) return;
-  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = (unsigned char)stim_flag(st, f, t0 + j * dt);
+  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, t0 + j * dt);
 }
 
 // ----------------------------------------------------------------------------
@@ -178,7 +190,9 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
   const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   for (int j = 0; j < n0D; j++) {
-    const int on = flags[(long long)f * n0D + j];
+    const unsigned char fl = flags[(long long)f * n0D + j];
+    if (!(fl & FLAG_COMPUTE)) continue;
+    const int on = fl & FLAG_STIM;
     const bool clamp = st.mode == 1 && on && inset;
     if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[f] : pbase;
     if (alg_transfer != nullptr && j == n0D - 1) {

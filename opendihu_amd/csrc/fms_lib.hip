@@ -95,6 +95,7 @@ struct FmsSolver {
   int log_cap = 0;
   double *d_onset = nullptr, *d_dur = nullptr, *d_amp = nullptr;
   int stim_param = 0;
+  int only_if_stimulated = 0;
   int fe_rows = 1, fe_cols = 1;
 
   // spikes
@@ -158,6 +159,7 @@ struct FmsSolver {
     s.duration = d_dur;
     s.amplitude = d_amp;
     s.stim_param = stim_param;
+    s.only_if_stimulated = only_if_stimulated;
     return s;
   }
 };
@@ -446,6 +448,7 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
   chk(dalloc(&s->d_pf, F));
   chk(dalloc(&s->d_params, (size_t)F * (Model::NP > 0 ? Model::NP : 1)));
   chk(dalloc(&s->d_vm, (size_t)N));
+  chk(dalloc(&s->d_hasbeen, (size_t)F));
   chk(hipMalloc(&s->d_gates, (size_t)(Model::NS - 1) * N * gate_bytes(s) + 8) == hipSuccess ? 0 : FMS_ERR_HIP);
   if (s->thomas == FMS_THOMAS_SERIAL && s->kernels == FMS_KERNELS_SPLIT) {
     chk(dalloc(&s->d_cA, (size_t)N));
@@ -469,6 +472,7 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
   chk(h2d(s->d_sb, zeros.data(), F, s->stream));
   chk(h2d(s->d_se, zeros.data(), F, s->stream));
   chk(h2d(s->d_order, s->h_order.data(), F, s->stream));
+  if (!rc && hipMemsetAsync(s->d_hasbeen, 0, F, s->stream) != hipSuccess) rc = FMS_ERR_HIP;
   chk(h2d(s->d_h, c->element_lengths, (size_t)(N - F), s->stream));
   chk(h2d(s->d_pf, c->prefactor, F, s->stream));
   chk(h2d(s->d_params, params.data(), params.size(), s->stream));
@@ -534,7 +538,7 @@ int fms_set_stimulation(FmsSolver *s, const FmsStimulation *st) {
     if (!s->d_fe) {
       chk(dalloc(&s->d_mu, F)); chk(dalloc(&s->d_joff, F)); chk(dalloc(&s->d_jcnt, F)); chk(dalloc(&s->d_jidx, F));
       chk(dalloc(&s->d_freq, F)); chk(dalloc(&s->d_rep, F)); chk(dalloc(&s->d_beg, F)); chk(dalloc(&s->d_last, F));
-      chk(dalloc(&s->d_curjit, F)); chk(dalloc(&s->d_curr, F)); chk(dalloc(&s->d_hasbeen, F));
+      chk(dalloc(&s->d_curjit, F)); chk(dalloc(&s->d_curr, F));
       chk(dalloc(&s->d_log_count, 1));
     } else {
       HIP_TRY(hipFree(s->d_fe));
@@ -726,6 +730,10 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
     case FMS_TUNE_STEPS_PER_LAUNCH:
       if (value < 1) return fail(FMS_ERR_ARGUMENT, "steps per launch must be >= 1");
       s->tune_chunk = value;
+      return 0;
+    case FMS_TUNE_ONLY_IF_STIMULATED:
+      if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "onlyComputeIfHasBeenStimulated is 0 or 1");
+      s->only_if_stimulated = value;
       return 0;
     case FMS_TUNE_RESIDENT_GATES:
       if (value < -1 || value > 1) return fail(FMS_ERR_ARGUMENT, "resident must be -1, 0 or 1");

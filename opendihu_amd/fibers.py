@@ -151,6 +151,11 @@ class FiberBatch:
         self._chk(self.lib.fms_advance_schedule(self.handle, L.as_ptr(steps, L._dp), steps.shape[0], int(n0D)),
                   "fms_advance_schedule")
 
+    def set_only_compute_if_stimulated(self, on: bool) -> None:
+        """onlyComputeIfHasBeenStimulated: skip a fibre's reaction until its first stimulation"""
+        self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_ONLY_IF_STIMULATED, int(bool(on))),
+                  "fms_set_tuning")
+
     def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None) -> None:
         """fused-kernel schedule: split steps per launch; gates in registers (1), streamed (0) or auto (-1)"""
         if steps_per_launch is not None:

@@ -127,6 +127,9 @@ typedef struct {
   int stim_param;                /* index of the parameter that carries the current */
   /* spike detector (extension): upward crossing of the threshold */
   double spike_threshold;
+  /* onlyComputeIfHasBeenStimulated (compute.tpp:201-205), applied per fibre:
+     the reaction of a fibre runs from the Heun step of its first stimulation on */
+  int only_if_stimulated;
 } fmso_problem;
 
 typedef struct {
@@ -221,7 +224,8 @@ static void fmso_compute0D_fiber(const fmso_problem *p, fmso_state *s, int f, lo
   for (int k = 0; k < NP; k++) prm[k] = prm_on[k] = p->params[(long)f * NP + k];
   for (int j = 0; j < n0D; j++) {
     const double t = t0 + j * dt;
-    const int on = flags[j];
+    if (!(flags[j] & 2)) continue;
+    const int on = flags[j] & 1;
     if (p->stim_mode == FMSO_STIM_CURRENT && NP > 0)
       prm_on[p->stim_param] = on ? p->stim_amplitude[f] : prm[p->stim_param];
     for (int i = 0; i < n; i++) {
@@ -306,9 +310,15 @@ static void fmso_fiber_flags(const fmso_problem *p, fmso_state *s, int f, double
 {
   for (int j = 0; j < n0D; j++) {
     const double t = t0 + j * dt;
-    if (p->stim_mode == FMSO_STIM_FIRING_TABLE) flags[j] = (unsigned char)fmso_stim_check(p, s, f, t);
-    else if (p->stim_mode == FMSO_STIM_CURRENT) flags[j] = (unsigned char)fmso_current_on(p, f, t);
-    else flags[j] = 0;
+    int stim = 0;
+    if (p->stim_mode == FMSO_STIM_FIRING_TABLE) {
+      stim = fmso_stim_check(p, s, f, t);
+    } else if (p->stim_mode == FMSO_STIM_CURRENT) {
+      stim = fmso_current_on(p, f, t);
+      if (stim) s->has_been_stimulated[f] = 1;
+    }
+    const int compute = !p->only_if_stimulated || s->has_been_stimulated[f];
+    flags[j] = (unsigned char)((stim ? 1 : 0) | (compute ? 2 : 0));
   }
 }
 

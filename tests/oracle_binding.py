@@ -31,6 +31,7 @@ class fmso_problem(ctypes.Structure):
         ("jitter", _dp), ("jitter_offset", _ip), ("jitter_count", _ip),
         ("stim_onset", _dp), ("stim_duration", _dp), ("stim_amplitude", _dp), ("stim_param", ctypes.c_int),
         ("spike_threshold", ctypes.c_double),
+        ("only_if_stimulated", ctypes.c_int),
     ]
 
 
@@ -189,6 +190,9 @@ class OracleBatch:
         p.stim_begin, p.stim_end = P(a["sb"], _ip), P(a["se"], _ip)
         p.stim_onset, p.stim_duration, p.stim_amplitude = P(a["on"], _dp), P(a["du"], _dp), P(a["am"], _dp)
         p.stim_param = stim_param
+
+    def set_only_compute_if_stimulated(self, on: bool) -> None:
+        self.prob.only_if_stimulated = int(bool(on))
 
     def enable_spike_detection(self, threshold=0.0):
         self.prob.spike_threshold = threshold

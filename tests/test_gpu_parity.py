@@ -256,3 +256,21 @@ def test_algebraics_for_transfer_fused_equals_split():
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert np.all(np.isfinite(out[0][1])) and np.any(out[0][1] != 0)
+
+
+@pytest.mark.parametrize("kernels", ["split", "fused"])
+def test_only_compute_if_stimulated(kernels):
+    """onlyComputeIfHasBeenStimulated: fibres keep their initial state until their
+    first stimulation (firing table), then evolve; bit-exact vs the oracle"""
+    mesh, _ = uneven(12, lo=200, hi=600, seed=9)
+    g = _gpu_batch(mesh, scheme="IE", kernels=kernels, thomas="serial")
+    o = OracleBatch(**mesh, scheme="IE")
+    for b in (g, o):
+        _firing_setup(b, mesh, width=4)
+        b.set_only_compute_if_stimulated(True)
+        b.advance(0.0, 2e-3, 400)
+    np.testing.assert_array_equal(g.get_states(), o.get_states())
+    gates = o.get_state(1)
+    off = np.concatenate([[0], np.cumsum(mesh["n_nodes"])])
+    untouched = [f for f in range(12) if np.all(gates[off[f]:off[f + 1]] == 0.05)]
+    assert 0 < len(untouched) < 12, "some fibres must still be unstimulated at t = 0.8 ms"
