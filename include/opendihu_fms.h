@@ -153,6 +153,26 @@ int fms_record_event(FmsSolver *solver, int slot);
 int fms_event_elapsed_ms(FmsSolver *solver, int from_slot, int to_slot, double *ms);
 int fms_launch_counts(FmsSolver *solver, long long *fused, long long *reaction, long long *thomas);
 
+/* per-fibre parameters [n_fibers*n_params] and prefactors [n_fibers] (NULL keeps them) */
+int fms_set_parameters(FmsSolver *solver, const double *params, const double *prefactor);
+
+/* sensor nodes: n_sensors fibre-local node numbers per fibre, [n_fibers*n_sensors];
+   Vm at the sensors is recorded after every split step of fms_adjoint_forward */
+int fms_set_sensors(FmsSolver *solver, const int *sensor_nodes, int n_sensors);
+
+/* Discrete adjoint (Hodgkin-Huxley model, FP64, current-pulse or no stimulation).
+   fms_adjoint_forward runs n_steps split steps from the current state
+   (steps[k*4 + {0..3}] = {t, dt_split, dt0D, dt1D}), stores the state every
+   checkpoint_interval steps and returns the sensor traces
+   traces[(k*n_fibers + f)*n_sensors + j] (host buffer, may be NULL).
+   fms_adjoint_backward takes dL/dtraces in the same layout and returns
+   grad_params[f*3 + {0,1,2}] = dL/d(g_Na, g_K, g_L) and grad_prefactor[f] = dL/dprefactor
+   (dL/dsigma = grad_prefactor / (Am*Cm)). It recomputes each checkpoint segment
+   with a tape and leaves the solver's states unspecified. */
+int fms_adjoint_forward(FmsSolver *solver, const double *steps, int n_steps, int n0D, int checkpoint_interval,
+                        double *traces);
+int fms_adjoint_backward(FmsSolver *solver, const double *dtraces, double *grad_params, double *grad_prefactor);
+
 /* ---- reference plugin boundary ---- */
 typedef struct {
   int n_fibers_to_compute;

@@ -89,6 +89,7 @@ EXPORTED_SYMBOLS = [
     "fms_get_algebraics", "fms_set_algebraics_for_transfer", "fms_total_nodes",
     "fms_enable_spike_detection", "fms_get_spikes", "fms_get_stimulation_log", "fms_schedule",
     "fms_last_error", "fms_set_tuning", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
+    "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain",
 ]
 
@@ -133,6 +134,10 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_event_elapsed_ms.argtypes = [v, i, i, _dp]
     lib.fms_launch_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_longlong)]
+    lib.fms_set_parameters.argtypes = [v, _dp, _dp]
+    lib.fms_set_sensors.argtypes = [v, _ip, i]
+    lib.fms_adjoint_forward.argtypes = [v, _dp, i, i, i, _dp]
+    lib.fms_adjoint_backward.argtypes = [v, _dp, _dp, _dp]
     lib.fms_reference_set_dims.argtypes = [ctypes.POINTER(FmsReferenceDims)]
     lib.initializeArrays.argtypes = [_dp, _ip, _ip, ctypes.c_char_p, _dp, _ip, _ip, _dp, _dp, _dp, _dp]
     lib.initializeArrays.restype = None

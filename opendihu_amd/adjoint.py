@@ -1,0 +1,100 @@
+"""Gradients of a Vm-trace misfit with respect to per-fibre conductances and
+conductivity, through the hand-written discrete adjoint of the split step
+(opendihu_amd/csrc/fms_adjoint.h), exposed as a torch.autograd.Function.
+
+    theta[f] = (g_Na, g_K, g_L, sigma)          per fibre, FP64
+    traces = MonodomainTraces.apply(theta, problem)   (n_steps, n_fibers, n_sensors)
+
+``problem`` (:class:`AdjointProblem`) holds the fibre batch (Hodgkin-Huxley,
+current-pulse stimulation), the initial states, the split-step table, the
+sensor nodes and the checkpoint interval. Forward and backward run on the GPU;
+the tensors crossing the autograd boundary are CPU float64 tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .fibers import FiberBatch
+
+
+class AdjointProblem:
+    def __init__(self, batch: FiberBatch, steps, sensor_nodes, checkpoint_interval=100, n0D=1, am=500.0, cm=1.0):
+        if batch.model_name != "hodgkin_huxley_1952":
+            raise ValueError("the adjoint is implemented for the Hodgkin-Huxley model")
+        self.batch = batch
+        self.steps = np.ascontiguousarray(steps, np.float64).reshape(-1, 4)
+        self.n_steps = self.steps.shape[0]
+        self.sensor_nodes = np.ascontiguousarray(sensor_nodes, np.int32).reshape(batch.n_fibers, -1)
+        self.n_sensors = self.sensor_nodes.shape[1]
+        self.interval = int(checkpoint_interval)
+        self.n0D = int(n0D)
+        self.am_cm = am * cm
+        self.initial = batch.get_states()
+        self.base_params = batch.parameters.copy()
+        lib = batch.lib
+        L.check(lib, lib.fms_set_sensors(batch.handle, L.as_ptr(self.sensor_nodes, L._ip), self.n_sensors),
+                "fms_set_sensors")
+
+    def set_theta(self, theta: np.ndarray) -> None:
+        theta = np.asarray(theta, np.float64).reshape(self.batch.n_fibers, 4)
+        params = self.base_params.copy()
+        params[:, 1:4] = theta[:, 0:3]
+        pf = np.ascontiguousarray(theta[:, 3] / self.am_cm)
+        params = np.ascontiguousarray(params)
+        L.check(self.batch.lib, self.batch.lib.fms_set_parameters(self.batch.handle, L.as_ptr(params, L._dp),
+                                                                  L.as_ptr(pf, L._dp)), "fms_set_parameters")
+
+    def forward(self, theta) -> np.ndarray:
+        self.set_theta(theta)
+        for k in range(self.batch.n_states):
+            self.batch.set_state(k, self.initial[k])
+        out = np.empty((self.n_steps, self.batch.n_fibers, self.n_sensors))
+        lib = self.batch.lib
+        L.check(lib, lib.fms_adjoint_forward(self.batch.handle, L.as_ptr(self.steps, L._dp), self.n_steps, self.n0D,
+                                             self.interval, L.as_ptr(out, L._dp)), "fms_adjoint_forward")
+        return out
+
+    def backward(self, dtraces) -> np.ndarray:
+        """dL/dtheta for the last forward; theta columns (g_Na, g_K, g_L, sigma)"""
+        d = np.ascontiguousarray(dtraces, np.float64)
+        F = self.batch.n_fibers
+        gp = np.empty((F, 3))
+        gpf = np.empty(F)
+        lib = self.batch.lib
+        L.check(lib, lib.fms_adjoint_backward(self.batch.handle, L.as_ptr(d, L._dp), L.as_ptr(gp, L._dp),
+                                              L.as_ptr(gpf, L._dp)), "fms_adjoint_backward")
+        return np.concatenate([gp, (gpf / self.am_cm)[:, None]], axis=1)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class MonodomainTraces:
+    """torch.autograd.Function: theta (n_fibers, 4) -> sensor traces (n_steps, n_fibers, n_sensors)"""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, theta, problem: AdjointProblem):
+        if cls._fn is None:
+            torch = _torch()
+
+            class _Fn(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, th, prob):
+                    ctx.prob = prob
+                    out = prob.forward(th.detach().cpu().numpy())
+                    return torch.from_numpy(out)
+
+                @staticmethod
+                def backward(ctx, g):
+                    grad = ctx.prob.backward(g.detach().cpu().contiguous().numpy())
+                    return torch.from_numpy(grad), None
+
+            cls._fn = _Fn
+        return cls._fn.apply(theta, problem)

@@ -28,6 +28,23 @@ struct DevSpikes {
   int step0;            // global index of the step before this launch's first step
 };
 
+// sensor traces and the adjoint tape (both optional)
+struct DevTrace {
+  const int *sensor_slot;   // [N] sensor number of each node, -1 elsewhere; nullptr: no traces
+  double *trace;            // [K][F][n_sensors], this launch's first step first
+  int n_sensors;
+  double *tape;             // [K][2*NS+1][N]: y before half 1 (NS), Vm before diffusion (1),
+                            // y after it (NS); nullptr: none
+};
+
+template <int NS, class G>
+__device__ __forceinline__ void tape_store(const DevTrace &tr, long long N, int s, int slot0, long long i, double V,
+                                           const G *g, int nstore) {
+  double *t = tr.tape + ((long long)s * (2 * NS + 1) + slot0) * N + i;
+  t[0] = V;
+  for (int k = 1; k < nstore; k++) t[(long long)k * N] = (double)g[k - 1];
+}
+
 template <class M, class G, class E>
 __device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, double pbase, const DevStim &st,
                                                 bool inset, const unsigned char *flags, double t0, double dt,
@@ -47,7 +64,7 @@ template <class M, class G, class E, int T, int MMAX, int NPT>
 __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *__restrict__ vm,
                                              G *__restrict__ gates, const double *__restrict__ steps, int K,
                                              int n0D, double vstim, int ie, int serial, const int *__restrict__ order,
-                                             DevSpikes spk) {
+                                             DevSpikes spk, DevTrace tr) {
   extern __shared__ double sh[];
   const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x;
@@ -104,7 +121,9 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         const int i = tid + q * T;
         if (i < n) {
           double V = lv[i];
+          if (tr.tape) tape_store<M::NS>(tr, N, 0, 0, off + i, V, greg[q], M::NS);
           fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
+          if (tr.tape) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, greg[q], 1);
           lv[i] = V;
         }
       }
@@ -114,7 +133,9 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
 #pragma unroll
         for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + off + i];
         double V = lv[i];
+        if (tr.tape) tape_store<M::NS>(tr, N, 0, 0, off + i, V, g, M::NS);
         fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
+        if (tr.tape) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, g, 1);
         lv[i] = V;
 #pragma unroll
         for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = g[k];
@@ -148,7 +169,12 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     __syncthreads();
 
     auto node = [&](int i, double &V, G *g, double &vp, int &sf, int &sc) {
+      if (tr.tape) tape_store<M::NS>(tr, N, s, M::NS + 1, off + i, V, g, M::NS);
       fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim);
+      if (tr.sensor_slot) {
+        const int slot = tr.sensor_slot[off + i];
+        if (slot >= 0) tr.trace[((long long)s * fb.n_fibers + f) * tr.n_sensors + slot] = V;
+      }
       if (spk.enabled) {
         if (vp < spk.threshold && V >= spk.threshold) {
           sc++;
@@ -156,9 +182,12 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         }
         vp = V;
       }
-      if (more)
+      if (more) {
+        if (tr.tape) tape_store<M::NS>(tr, N, s + 1, 0, off + i, V, g, M::NS);
         fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D,
                                  vstim);
+        if (tr.tape) tape_store<M::NS>(tr, N, s + 1, M::NS, off + i, V, g, 1);
+      }
     };
 
     if (NPT > 0) {

@@ -311,7 +311,9 @@ __global__ __launch_bounds__(64) void k_thomas_serial(DevFibers fb, double *vm, 
 // red: 8*2T doubles of LDS scratch
 // Requires n >= 2 and ceil(n / min(T, n/2)) <= MMAX. Ends with a barrier.
 // ----------------------------------------------------------------------------
-template <int T, int MMAX>
+// GIVEN_RHS: lv holds the right-hand side itself (adjoint solves) instead of the
+// Vm from which the right-hand side B*Vm is assembled.
+template <int T, int MMAX, bool GIVEN_RHS = false>
 __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double *h, double prefactor, double dt,
                                                 bool ie, double *red, int tid) {
   constexpr int R = 2 * T;
@@ -329,6 +331,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
       if (k < cnt) {
         double a, b, c, d;
         assemble_row(lo + k, n, lv, h, prefactor, dt, ie, a, b, c, d);
+        if (GIVEN_RHS) d = lv[lo + k];
         if (k < 2) {
           aa[k] = a / b;
           cc[k] = c / b;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "../../include/opendihu_fms.h"
+#include "fms_adjoint.h"
 #include "fms_fused.h"
 #include "fms_kernels.h"
 
@@ -113,6 +115,18 @@ struct FmsSolver {
   hipEvent_t events[8] = {};
   long long n_launch_fused = 0, n_launch_reaction = 0, n_launch_thomas = 0;
 
+  // sensors, traces and the adjoint (checkpoints every ckpt_interval steps, tape of one segment)
+  int n_sensors = 0;
+  int *d_sensor_nodes = nullptr;   // [F][S] fibre-local
+  int *d_sensor_slot = nullptr;    // [N]
+  double *d_trace = nullptr, *d_dtrace = nullptr;
+  long long trace_cap = 0;
+  double *d_ckpt = nullptr, *d_tape = nullptr, *d_lam = nullptr, *d_gacc = nullptr, *d_pfbar = nullptr,
+         *d_grad = nullptr;
+  long long ckpt_cap = 0, tape_cap = 0;
+  std::vector<double> adj_steps;   // host copy of the last adjoint forward's step table
+  int adj_n0D = 1, adj_interval = 0, adj_K = 0;
+
   // algebraics for transfer
   int *d_alg_idx = nullptr;
   int n_alg_idx = 0;
@@ -172,7 +186,8 @@ void free_all(FmsSolver *s) {
                   s->d_joff, s->d_jcnt, s->d_jidx, s->d_freq, s->d_rep, s->d_beg, s->d_jit, s->d_last,
                   s->d_curjit, s->d_curr, s->d_hasbeen, s->d_log_count, s->d_log_fiber, s->d_log_time,
                   s->d_onset, s->d_dur, s->d_amp, s->d_vprev, s->d_sfirst, s->d_scount, s->d_alg_idx,
-                  s->d_alg};
+                  s->d_alg, s->d_sensor_nodes, s->d_sensor_slot, s->d_trace, s->d_dtrace, s->d_ckpt, s->d_tape,
+                  s->d_lam, s->d_gacc, s->d_pfbar, s->d_grad};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->events)
@@ -247,7 +262,7 @@ int run_split(FmsSolver *s, const double *steps, int K, int n0D) {
 }
 
 template <class G, class E, int NPT, int MMAX>
-int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D) {
+int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr) {
   constexpr int T = FUSED_T;
   fms::DevSpikes spk;
   spk.enabled = s->spikes;
@@ -260,7 +275,7 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D) {
   hipLaunchKernelGGL((fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->stream, s->fibers(),
                      s->stim(), s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim,
                      s->scheme == FMS_IMPLICIT_EULER ? 1 : 0, s->thomas == FMS_THOMAS_SERIAL ? 1 : 0, s->d_order,
-                     spk);
+                     spk, tr);
   HIP_TRY(hipGetLastError());
   s->step_counter += K;
   s->n_launch_fused++;
@@ -271,16 +286,16 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D) {
 // registers (small models), streamed from HBM otherwise; the partitioned
 // solve needs ceil(n / min(T, n/2)) <= MMAX rows per thread
 template <class G, class E>
-int launch_fused(FmsSolver *s, const double *d_steps, int K, int n0D) {
+int launch_fused(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr = fms::DevTrace{}) {
   constexpr int T = FUSED_T;
   const int npt = (s->nmax + T - 1) / T;
   if constexpr (Model::NS <= 8) {
     const bool resident = s->tune_resident != 0;
-    if (resident && npt <= 4) return launch_fused_npt<G, E, 4, 8>(s, d_steps, K, n0D);
-    if (resident && npt <= 8) return launch_fused_npt<G, E, 8, 8>(s, d_steps, K, n0D);
+    if (resident && npt <= 4) return launch_fused_npt<G, E, 4, 8>(s, d_steps, K, n0D, tr);
+    if (resident && npt <= 8) return launch_fused_npt<G, E, 8, 8>(s, d_steps, K, n0D, tr);
   }
-  if (npt <= 8) return launch_fused_npt<G, E, 0, 8>(s, d_steps, K, n0D);
-  return launch_fused_npt<G, E, 0, 16>(s, d_steps, K, n0D);
+  if (npt <= 8) return launch_fused_npt<G, E, 0, 8>(s, d_steps, K, n0D, tr);
+  return launch_fused_npt<G, E, 0, 16>(s, d_steps, K, n0D, tr);
 }
 
 template <class G, class E>
@@ -775,3 +790,203 @@ int fms_launch_counts(FmsSolver *s, long long *fused, long long *reaction, long 
 
 #include "fms_schedule.h"
 #include "fms_reference_abi.h"
+
+// ===========================================================================
+// parameters, sensors and the discrete adjoint
+// ===========================================================================
+namespace {
+
+template <class T> int ensure(T **p, long long &cap, long long n) {
+  if (cap >= n && *p) return 0;
+  if (*p) HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  cap = 0;
+  if (dalloc(p, (size_t)n)) return FMS_ERR_HIP;
+  cap = n;
+  return 0;
+}
+
+int upload_steps(FmsSolver *s, const double *steps, int K) {
+  if (s->steps_cap < 4 * (K + 1)) {
+    if (s->d_steps) HIP_TRY(hipFree(s->d_steps));
+    s->d_steps = nullptr;
+    s->steps_cap = 4 * (K + 1);
+    if (dalloc(&s->d_steps, (size_t)s->steps_cap)) return FMS_ERR_HIP;
+  }
+  std::vector<double> tab(4 * (size_t)(K + 1));
+  memcpy(tab.data(), steps, sizeof(double) * 4 * K);
+  for (int q = 0; q < 4; q++) tab[4 * (size_t)K + q] = steps[4 * (size_t)(K - 1) + q];
+  HIP_TRY(hipMemcpyAsync(s->d_steps, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  return 0;
+}
+
+int copy_state(FmsSolver *s, double *dst, bool to_ckpt) {
+  const size_t n = (size_t)s->N;
+  const size_t gb = (size_t)(Model::NS - 1) * n * sizeof(double);
+  if (to_ckpt) {
+    HIP_TRY(hipMemcpyAsync(dst, s->d_vm, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(dst + n, s->d_gates, gb, hipMemcpyDeviceToDevice, s->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(s->d_vm, dst, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_gates, dst + n, gb, hipMemcpyDeviceToDevice, s->stream));
+  }
+  return 0;
+}
+
+template <class E> int adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_params, double *pf_bar) {
+  if constexpr (!std::is_same<Model, fms::ModelHH>::value) {
+    (void)s; (void)dtraces; (void)grad_params; (void)pf_bar;
+    return fail(FMS_ERR_UNSUPPORTED, "the adjoint is implemented for the Hodgkin-Huxley model");
+  } else {
+    const long long N = s->N;
+    const int F = s->F, S = s->n_sensors, K = s->adj_K, Kc = s->adj_interval, n0D = s->adj_n0D;
+    const int ie = s->scheme == FMS_IMPLICIT_EULER;
+    long long cap = 0;
+    int rc;
+    if ((rc = ensure(&s->d_dtrace, cap, (long long)K * F * S))) return rc;
+    HIP_TRY(hipMemcpyAsync(s->d_dtrace, dtraces, (size_t)K * F * S * sizeof(double), hipMemcpyHostToDevice,
+                           s->stream));
+    long long c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+    if ((rc = ensure(&s->d_lam, c1, 4 * N)) || (rc = ensure(&s->d_gacc, c2, 3 * N)) ||
+        (rc = ensure(&s->d_pfbar, c3, F)) || (rc = ensure(&s->d_grad, c4, 3 * (long long)F)))
+      return rc;
+    HIP_TRY(hipMemsetAsync(s->d_lam, 0, 4 * N * sizeof(double), s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_gacc, 0, 3 * N * sizeof(double), s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_pfbar, 0, F * sizeof(double), s->stream));
+    if ((rc = ensure(&s->d_tape, s->tape_cap, (long long)Kc * 9 * N))) return rc;
+    if ((rc = upload_steps(s, s->adj_steps.data(), K))) return rc;
+    const int saved_spikes = s->spikes;
+    s->spikes = 0;
+    const int nseg = (K + Kc - 1) / Kc;
+    fms::DevTape tp{s->d_tape, N};
+    const int threads = 256;
+    const unsigned blocks = (unsigned)((N + threads - 1) / threads);
+    const size_t lds = (size_t)(16 * PART_T + 2 * s->nmax + PART_T) * sizeof(double);
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+      const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
+      if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, false))) break;
+      fms::DevTrace tr{};
+      tr.tape = s->d_tape;
+      const long long save = s->step_counter;
+      if ((rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr))) break;
+      s->step_counter = save;
+      for (int q = kc - 1; q >= 0; q--) {
+        const int k = k0 + q;
+        const double *st = s->adj_steps.data() + 4 * (size_t)k;
+        const double t = st[0], dts = st[1], dt0 = st[2], dt1 = st[3];
+        hipLaunchKernelGGL(fms::k_adj_sensors, dim3((F * S + 255) / 256), dim3(256), 0, s->stream, s->fibers(),
+                           s->d_sensor_nodes, S, s->d_dtrace, k, s->d_lam);
+        hipLaunchKernelGGL((fms::k_adj_heun<E>), dim3(blocks), dim3(threads), 0, s->stream, s->fibers(), s->stim(),
+                           tp, q, 1, t + 0.5 * dts, dt0, n0D, s->d_lam, s->d_gacc);
+        hipLaunchKernelGGL((fms::k_adj_thomas<PART_T, 16>), dim3(F), dim3(PART_T), lds, s->stream, s->fibers(), tp, q,
+                           dt1, ie, s->d_lam, s->d_pfbar);
+        hipLaunchKernelGGL((fms::k_adj_heun<E>), dim3(blocks), dim3(threads), 0, s->stream, s->fibers(), s->stim(),
+                           tp, q, 0, t, dt0, n0D, s->d_lam, s->d_gacc);
+        if (hipGetLastError() != hipSuccess) {
+          rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
+          break;
+        }
+      }
+      if (rc) break;
+    }
+    s->spikes = saved_spikes;
+    if (rc) return rc;
+    hipLaunchKernelGGL(fms::k_adj_reduce, dim3(F), dim3(256), 0, s->stream, s->fibers(), s->d_gacc, s->d_grad);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (grad_params) HIP_TRY(hipMemcpy(grad_params, s->d_grad, 3 * (size_t)F * sizeof(double), hipMemcpyDeviceToHost));
+    if (pf_bar) HIP_TRY(hipMemcpy(pf_bar, s->d_pfbar, (size_t)F * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fms_set_parameters(FmsSolver *s, const double *params, const double *prefactor) {
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (params && Model::NP > 0)
+    HIP_TRY(hipMemcpy(s->d_params, params, (size_t)s->F * Model::NP * sizeof(double), hipMemcpyHostToDevice));
+  if (prefactor) HIP_TRY(hipMemcpy(s->d_pf, prefactor, (size_t)s->F * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fms_set_sensors(FmsSolver *s, const int *sensor_nodes, int n_sensors) {
+  if (!s || n_sensors < 0 || (n_sensors > 0 && !sensor_nodes)) return fail(FMS_ERR_ARGUMENT, "bad sensors");
+  const int F = s->F;
+  std::vector<int> slot((size_t)s->N, -1);
+  for (int f = 0; f < F; f++) {
+    const int n = s->h_off[f + 1] - s->h_off[f];
+    for (int j = 0; j < n_sensors; j++) {
+      const int i = sensor_nodes[(size_t)f * n_sensors + j];
+      if (i < 0 || i >= n) return fail(FMS_ERR_ARGUMENT, "sensor node outside its fibre");
+      if (slot[(size_t)s->h_off[f] + i] >= 0) return fail(FMS_ERR_ARGUMENT, "duplicate sensor node");
+      slot[(size_t)s->h_off[f] + i] = j;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->d_sensor_nodes) HIP_TRY(hipFree(s->d_sensor_nodes));
+  if (s->d_sensor_slot) HIP_TRY(hipFree(s->d_sensor_slot));
+  s->d_sensor_nodes = nullptr;
+  s->d_sensor_slot = nullptr;
+  s->n_sensors = n_sensors;
+  if (n_sensors == 0) return 0;
+  if (dalloc(&s->d_sensor_nodes, (size_t)F * n_sensors) || dalloc(&s->d_sensor_slot, (size_t)s->N)) return FMS_ERR_HIP;
+  HIP_TRY(hipMemcpy(s->d_sensor_nodes, sensor_nodes, (size_t)F * n_sensors * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_sensor_slot, slot.data(), (size_t)s->N * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fms_adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int checkpoint_interval,
+                        double *traces) {
+  if (!s || !steps || n_steps <= 0 || n0D < 1 || checkpoint_interval < 1)
+    return fail(FMS_ERR_ARGUMENT, "bad adjoint forward arguments");
+  if (s->n_sensors <= 0) return fail(FMS_ERR_STATE, "call fms_set_sensors first");
+  if (s->precision != FMS_FP64) return fail(FMS_ERR_UNSUPPORTED, "the adjoint runs in FP64");
+  if (s->stim_mode == FMS_STIM_FIRING_TABLE)
+    return fail(FMS_ERR_UNSUPPORTED, "the Vm clamp of firing-table stimulation is not differentiable; use current pulses");
+  if (s->only_if_stimulated) return fail(FMS_ERR_UNSUPPORTED, "onlyComputeIfHasBeenStimulated is not differentiable");
+  const long long N = s->N;
+  const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
+  const int nseg = (K + Kc - 1) / Kc;
+  int rc;
+  if ((rc = ensure(&s->d_trace, s->trace_cap, (long long)K * F * S))) return rc;
+  if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)nseg * Model::NS * N))) return rc;
+  if (s->flags_cap < s->F * n0D) {
+    if (s->d_flags) HIP_TRY(hipFree(s->d_flags));
+    s->flags_cap = s->F * n0D;
+    if (dalloc(&s->d_flags, (size_t)s->flags_cap)) return FMS_ERR_HIP;
+  }
+  if ((rc = upload_steps(s, steps, K))) return rc;
+  s->adj_steps.assign(steps, steps + 4 * (size_t)K);
+  s->adj_n0D = n0D;
+  s->adj_interval = Kc;
+  s->adj_K = K;
+  for (int seg = 0; seg < nseg; seg++) {
+    const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
+    if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, true))) return rc;
+    fms::DevTrace tr{};
+    tr.sensor_slot = s->d_sensor_slot;
+    tr.trace = s->d_trace + (size_t)k0 * F * S;
+    tr.n_sensors = S;
+    if (s->exp_mode == FMS_EXP_APPROXIMATE) rc = launch_fused<double, fms::ExpApprox>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
+    else rc = launch_fused<double, fms::ExpLibm>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
+    if (rc) return rc;
+  }
+  if (traces) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(traces, s->d_trace, (size_t)K * F * S * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int fms_adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_params, double *grad_prefactor) {
+  if (!s || !dtraces) return fail(FMS_ERR_ARGUMENT, "null argument");
+  if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
+  if (s->exp_mode == FMS_EXP_APPROXIMATE) return adjoint_backward<fms::ExpApprox>(s, dtraces, grad_params, grad_prefactor);
+  return adjoint_backward<fms::ExpLibm>(s, dtraces, grad_params, grad_prefactor);
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""Discrete adjoint on the GPU: gradients of a Vm-trace misfit with respect to
+per-fibre (g_Na, g_K, g_L, sigma), checked against central finite differences
+of the same forward (h = 1e-6 relative), through the torch.autograd.Function.
+"""
+import numpy as np
+import pytest
+
+from cases import AM, CM, uneven
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n_fibers=4, n_steps=600, interval=100, scheme="CN", seed=6, lo=200, hi=400, n_sensors=8):
+    from opendihu_amd.adjoint import AdjointProblem
+    from opendihu_amd.fibers import FiberBatch
+    mesh, stim = uneven(n_fibers, lo=lo, hi=hi, seed=seed)
+    n = mesh["n_nodes"]
+    mid = n // 2
+    stim["stim_begin"] = (mid - 1).astype(np.int32)
+    stim["stim_end"] = (mid + 2).astype(np.int32)
+    stim["onset"] = np.random.default_rng(seed).uniform(0.0, 0.3, n_fibers)
+    b = FiberBatch(**mesh, scheme=scheme, kernels="fused", thomas="partitioned")
+    b.set_current_stimulation(**stim)
+    dt = 1e-3
+    steps = np.stack([np.arange(n_steps) * dt, np.full(n_steps, dt), np.full(n_steps, dt / 2), np.full(n_steps, dt)], 1)
+    sensors = np.stack([np.round(np.linspace(0, k - 1, n_sensors)).astype(np.int32) for k in n])
+    prob = AdjointProblem(b, steps, sensors, checkpoint_interval=interval)
+    rng = np.random.default_rng(seed + 1)
+    theta_true = np.stack([rng.uniform(100, 140, n_fibers), rng.uniform(30, 42, n_fibers),
+                           rng.uniform(0.2, 0.4, n_fibers), rng.uniform(3.0, 4.5, n_fibers)], 1)
+    theta0 = theta_true * rng.uniform(0.95, 1.05, theta_true.shape)
+    return prob, theta_true, theta0
+
+
+def _loss(prob, theta, target):
+    tr = prob.forward(theta)
+    return 0.5 * float(np.sum((tr - target) ** 2)), tr
+
+
+@pytest.mark.parametrize("scheme", ["CN", "IE"])
+def test_adjoint_matches_central_differences(scheme):
+    prob, theta_true, theta0 = _problem(scheme=scheme)
+    target = prob.forward(theta_true)
+    L0, tr = _loss(prob, theta0, target)
+    grad = prob.backward(tr - target)
+    assert L0 > 0 and np.all(np.isfinite(grad))
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(theta0.shape) * theta0
+    h = 1e-6
+    lp, _ = _loss(prob, theta0 + h * v, target)
+    lm, _ = _loss(prob, theta0 - h * v, target)
+    fd = (lp - lm) / (2 * h)
+    ad = float(np.sum(grad * v))
+    print(f"{scheme}: directional derivative adjoint {ad:.10e} FD {fd:.10e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) <= 1e-5 * abs(fd)
+    for f, q in [(0, 0), (1, 1), (2, 2), (3, 3), (0, 3)]:
+        e = np.zeros_like(theta0)
+        e[f, q] = h * theta0[f, q]
+        lp, _ = _loss(prob, theta0 + e, target)
+        lm, _ = _loss(prob, theta0 - e, target)
+        fdq = (lp - lm) / (2 * e[f, q])
+        print(f"  fibre {f} param {q}: adjoint {grad[f, q]:.8e} FD {fdq:.8e}")
+        assert abs(grad[f, q] - fdq) <= 1e-5 * max(abs(fdq), 1e-3 * np.max(np.abs(grad[:, q])))
+
+
+def test_adjoint_traces_match_forward_run():
+    prob, theta_true, _ = _problem(n_steps=300)
+    tr = prob.forward(theta_true)
+    b = prob.batch
+    for k in range(b.n_states):
+        b.set_state(k, prob.initial[k])
+    b.advance_schedule(prob.steps)
+    v = b.vm()
+    off = np.concatenate([[0], np.cumsum(b.n_nodes)[:-1]])
+    np.testing.assert_array_equal(tr[-1], v[off[:, None] + prob.sensor_nodes])
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    from opendihu_amd.adjoint import MonodomainTraces
+    prob, theta_true, theta0 = _problem(n_steps=300, interval=64)
+    target = torch.from_numpy(prob.forward(theta_true))
+    th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    loss = 0.5 * ((MonodomainTraces.apply(th, prob) - target) ** 2).sum()
+    loss.backward()
+    tr = prob.forward(theta0)
+    expect = prob.backward(tr - target.numpy())
+    np.testing.assert_allclose(th.grad.numpy(), expect, rtol=1e-12, atol=0)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    prob, theta_true, theta0 = _problem(n_steps=400, interval=400)
+    target = prob.forward(theta_true)
+    tr = prob.forward(theta0)
+    g1 = prob.backward(tr - target)
+    prob.interval = 37
+    tr = prob.forward(theta0)
+    g2 = prob.backward(tr - target)
+    np.testing.assert_array_equal(g1, g2)
