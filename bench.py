@@ -19,7 +19,11 @@ The JSON line carries:
   cpu_baseline     the FP64 CPU oracle (the reference's vc-path restatement) on the
                    host cores, over a bounded sample of the same workload
   mixed            the FP32-state / FP64-Vm form on the same workload
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1]
+  workloads        batched uneven HH under the autotuned schedule (FP64, mixed,
+                   spike-time error, CPU oracle), the adjoint's gradient cost
+                   against one forward solve, and the large mixed-precision
+                   Shorten batch (16384 uneven fibres) against its FP64 run
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1] [--no-extras]
 """
 from __future__ import annotations
 
@@ -131,6 +135,143 @@ def cpu_baseline(mesh, stim, scheme, model, dt, budget_s=12.0):
                        f"({el:.1f} s, FP64 oracle, {threads} OpenMP threads on {cpu})")
 
 
+def _spike_shift(first_a, first_b):
+    """largest difference in first-spike step over nodes that fired in either run"""
+    fired = (first_a >= 0) | (first_b >= 0)
+    if not fired.any():
+        return 0, 0
+    big = 1 << 30
+    a = np.where(first_a >= 0, first_a, big)
+    b = np.where(first_b >= 0, first_b, big)
+    return int(np.max(np.abs(a[fired] - b[fired]))), int(fired.sum())
+
+
+def bench_c3(cpu=True):
+    """Batched uneven HH (4096 fibres of 400-2000 nodes, CN, dt=1e-3 ms, 10 ms):
+    schedule chosen by the autotuner, FP64 and mixed, spike-time error of the
+    mixed run against the FP64 run, FP64 CPU oracle on a bounded sample."""
+    from cases import uneven
+    from opendihu_amd.fibers import FiberBatch
+    from opendihu_amd.build import HH_LIBRARY
+    dt, n_steps = 1e-3, 10000
+    mesh, stim = uneven(4096, seed=3)
+    trial = np.stack([np.arange(64) * dt, np.full(64, dt), np.full(64, dt), np.full(64, dt)], 1)
+    res = {}
+    for precision in ("fp64", "mixed"):
+        g = FiberBatch(**mesh, scheme="CN", precision=precision, kernels="fused", thomas="partitioned",
+                       library=HH_LIBRARY)
+        g.set_current_stimulation(**stim)
+        g.enable_spike_detection(0.0)
+        choice = g.autotune(trial, allow_split=True)
+        g.synchronize()
+        g.record_event(0)
+        w0 = time.perf_counter()
+        g.advance(0.0, dt, n_steps)
+        g.record_event(1)
+        g.synchronize()
+        wall = time.perf_counter() - w0
+        res[precision] = dict(value=g.n_total * n_steps / wall, ms_per_step=wall / n_steps * 1e3,
+                              event_ms=g.elapsed_ms(0, 1), schedule=choice, vm=g.vm(), first=g.spikes()[0])
+        g.close()
+    shift, fired = _spike_shift(res["mixed"]["first"], res["fp64"]["first"])
+    vm_err = float(np.max(np.abs(res["mixed"]["vm"] - res["fp64"]["vm"])))
+    out = dict(workload="hh_4096_uneven_400-2000_CN_dt1e-3_10ms", nodes=int(mesh["n_nodes"].sum()), steps=n_steps,
+               value_fp64=res["fp64"]["value"], value_mixed=res["mixed"]["value"],
+               ms_per_step_fp64=res["fp64"]["ms_per_step"], ms_per_step_mixed=res["mixed"]["ms_per_step"],
+               schedule_fp64={k: v for k, v in res["fp64"]["schedule"].items()},
+               schedule_mixed={k: v for k, v in res["mixed"]["schedule"].items()},
+               mixed_vs_fp64=dict(spike_time_error_steps=shift, nodes_fired=fired, vm_max_abs_err_mV=vm_err))
+    if cpu:
+        c = cpu_baseline(mesh, stim, "CN", "hh", dt, budget_s=10.0)
+        out["cpu_baseline"] = c
+        out["speedup_mixed_vs_cpu"] = res["mixed"]["value"] / c["value"]
+        out["speedup_fp64_vs_cpu"] = res["fp64"]["value"] / c["value"]
+    return out
+
+
+def bench_c4():
+    """Adjoint: HH on 256 fibres x 1000 nodes, 5 ms at dt=1e-3 ms (5000 split
+    steps), 32 sensors per fibre, checkpoint every 100 steps. Cost of the
+    gradient (recording forward + backward) against one plain forward solve,
+    and one directional central difference (h=1e-6) at full size."""
+    from cases import uneven, AM, CM
+    from opendihu_amd.fibers import FiberBatch
+    from opendihu_amd.adjoint import AdjointProblem
+    F, n_nodes, n_steps, dt = 256, 1000, 5000, 1e-3
+    mesh, stim = uneven(F, lo=n_nodes, hi=n_nodes, seed=41)
+    b = FiberBatch(**mesh, scheme="CN", kernels="fused", thomas="partitioned")
+    b.set_current_stimulation(**stim)
+    steps = np.stack([np.arange(n_steps) * dt, np.full(n_steps, dt), np.full(n_steps, dt), np.full(n_steps, dt)], 1)
+    sensors = np.tile(np.round(np.linspace(0, n_nodes - 1, 32)).astype(np.int32), (F, 1))
+    prob = AdjointProblem(b, steps, sensors, checkpoint_interval=100, am=AM, cm=CM)
+    rng = np.random.default_rng(42)
+    sigma = mesh["prefactor"] * AM * CM
+    theta_true = np.stack([rng.uniform(100, 140, F), rng.uniform(30, 42, F), rng.uniform(0.2, 0.4, F), sigma], 1)
+    theta0 = theta_true * rng.uniform(0.95, 1.05, theta_true.shape)
+    target = prob.forward(theta_true)
+    # plain forward solve (no traces, no checkpoints) at theta0
+    prob.set_theta(theta0)
+    for k in range(b.n_states):
+        b.set_state(k, prob.initial[k])
+    b.synchronize()
+    w0 = time.perf_counter()
+    b.advance_schedule(steps)
+    b.synchronize()
+    t_plain = time.perf_counter() - w0
+    w0 = time.perf_counter()
+    tr = prob.forward(theta0)
+    t_fwd = time.perf_counter() - w0
+    resid = tr - target
+    w0 = time.perf_counter()
+    grad = prob.backward(resid)
+    t_bwd = time.perf_counter() - w0
+    loss = lambda th: 0.5 * float(np.sum((prob.forward(th) - target) ** 2))  # noqa: E731
+    v = np.random.default_rng(43).standard_normal(theta0.shape) * theta0
+    h = 1e-6
+    fd = (loss(theta0 + h * v) - loss(theta0 - h * v)) / (2 * h)
+    ad = float(np.sum(grad * v))
+    b.close()
+    return dict(workload="hh_adjoint_256x1000_CN_dt1e-3_5ms_32sensors_ckpt100", steps=n_steps,
+                plain_forward_s=t_plain, recording_forward_s=t_fwd, backward_s=t_bwd,
+                gradient_cost_vs_forward=(t_fwd + t_bwd) / t_plain, backward_vs_forward=t_bwd / t_plain,
+                directional_derivative=dict(adjoint=ad, central_fd=fd, h=h, rel_err=abs(ad - fd) / abs(fd)),
+                note="wall times include the host copies of the traces (5000 x 256 x 32 doubles each way)")
+
+
+def bench_c5(steps=200, warmup=20):
+    """Large mixed-precision batch: Shorten on 16384 fibres of U{1000..1481}
+    nodes, 100 motor units (Poisson 10-40 Hz, every unit firing at t=0),
+    IE, dt=2.5e-5 ms; bounded to `steps` timed split steps; the mixed run's Vm
+    and spike-time error against the FP64 run after the same steps."""
+    from cases import c5
+    from opendihu_amd.fibers import FiberBatch
+    from opendihu_amd.build import SHORTEN_LIBRARY
+    dt = 2.5e-5
+    mesh, stim = c5(n_rows=steps + warmup + 1)
+    res = {}
+    for precision in ("mixed", "fp64"):
+        g = FiberBatch(**mesh, scheme="IE", precision=precision, kernels="fused", thomas="partitioned",
+                       library=SHORTEN_LIBRARY)
+        g.set_firing_table_stimulation(**stim)
+        g.enable_spike_detection(0.0)
+        g.advance(0.0, dt, warmup)
+        g.synchronize()
+        w0 = time.perf_counter()
+        g.advance(warmup * dt, dt, steps)
+        g.synchronize()
+        wall = time.perf_counter() - w0
+        res[precision] = dict(value=g.n_total * steps / wall, ms_per_step=wall / steps * 1e3, vm=g.vm(),
+                              first=g.spikes()[0])
+        g.close()
+    shift, fired = _spike_shift(res["mixed"]["first"], res["fp64"]["first"])
+    return dict(workload="shorten56_16384_uneven_1000-1481_IE_dt2.5e-5_100MU", nodes=int(mesh["n_nodes"].sum()),
+                steps=steps, warmup=warmup, value_mixed=res["mixed"]["value"], value_fp64=res["fp64"]["value"],
+                ms_per_step_mixed=res["mixed"]["ms_per_step"], ms_per_step_fp64=res["fp64"]["ms_per_step"],
+                mixed_vs_fp64=dict(vm_max_abs_err_mV=float(np.max(np.abs(res["mixed"]["vm"] - res["fp64"]["vm"]))),
+                                   spike_time_error_steps=shift, nodes_fired=fired,
+                                   after_steps=steps + warmup))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +281,9 @@ def main(argv=None):
     ap.add_argument("--kernels", default="fused", choices=["fused", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-HH (autotuned), adjoint and large mixed-precision Shorten lines")
+    ap.add_argument("--extras", default="c3,c4,c5", help="comma list of extra workloads to run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -188,6 +332,21 @@ def main(argv=None):
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mesh, stim, scheme, oracle_model, dt)
 
+    extras = {}
+    if not args.no_extras:
+        for name in [x for x in args.extras.split(",") if x]:
+            t0 = time.perf_counter()
+            if name == "c3":
+                extras[name] = bench_c3(cpu=not args.no_cpu_baseline)
+            elif name == "c4":
+                extras[name] = bench_c4()
+            elif name == "c5":
+                extras[name] = bench_c5()
+            else:
+                raise SystemExit(f"unknown extra workload {name}")
+            extras[name]["bench_wall_s"] = time.perf_counter() - t0
+            print(f"# {name} done in {extras[name]['bench_wall_s']:.1f} s", file=sys.stderr, flush=True)
+
     out = {
         "metric": "fiber-node-steps/s",
         "value": value,
@@ -219,6 +378,7 @@ def main(argv=None):
         },
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "workloads": extras,
     }
     print(json.dumps(out), flush=True)
     return 0

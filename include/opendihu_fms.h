@@ -145,7 +145,34 @@ int fms_last_error(char *buf, int len);
 /* onlyComputeIfHasBeenStimulated (0/1, default 0): a fibre's reaction runs from
    the Heun step of its first stimulation on; diffusion always runs */
 #define FMS_TUNE_ONLY_IF_STIMULATED 3
+#define FMS_TUNE_THREADS 4            /* threads per fibre workgroup of the fused kernel: 256 (default) or 128 */
+#define FMS_TUNE_GRAPH_STEPS 5        /* split kernels: steps per captured hipGraph, 0 = launch eagerly (default) */
 int fms_set_tuning(FmsSolver *solver, int key, int value);
+
+/* graphs captured and graph replays issued so far (split kernels) */
+int fms_graph_counts(FmsSolver *solver, long long *captures, long long *replays);
+
+/* Autotuner (no counterpart in the reference, which has one fixed CPU schedule
+   and an OpenMP-offload loop): times every schedule that gives bit-identical
+   states on this batch - fused kernel with gates resident or streamed, 128 or
+   256 threads per fibre, 64 or 512 steps per launch; with allow_split also the
+   split kernels launched eagerly or replayed as hipGraphs of 8 or 32 steps -
+   over the n_steps records of `steps` (4 doubles each, as
+   fms_advance_schedule), restores the states, stimulation state and spike
+   counters after every trial and keeps the fastest schedule. candidate_ms
+   (optional, `capacity` entries) receives each candidate's ms per step in the
+   order tried. */
+typedef struct {
+  int kernels;            /* FMS_KERNELS_SPLIT / FMS_KERNELS_FUSED */
+  int steps_per_launch;
+  int resident_gates;
+  int threads;
+  int graph_steps;
+  double ms_per_step;
+  int n_candidates;
+} FmsTuneResult;
+int fms_autotune(FmsSolver *solver, const double *steps, int n_steps, int n0D, int allow_split,
+                 FmsTuneResult *result, double *candidate_ms, int capacity);
 
 /* timing on the solver's stream: events in slots 0..7, elapsed time between
    two recorded slots, and the number of kernel launches issued so far */

@@ -19,10 +19,23 @@ FMS_THOMAS_SERIAL, FMS_THOMAS_PARTITIONED = 0, 1
 FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
 FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
 FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES, FMS_TUNE_ONLY_IF_STIMULATED = 1, 2, 3
+FMS_TUNE_THREADS, FMS_TUNE_GRAPH_STEPS = 4, 5
 
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)
 _ucp = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class FmsTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("kernels", ctypes.c_int),
+        ("steps_per_launch", ctypes.c_int),
+        ("resident_gates", ctypes.c_int),
+        ("threads", ctypes.c_int),
+        ("graph_steps", ctypes.c_int),
+        ("ms_per_step", ctypes.c_double),
+        ("n_candidates", ctypes.c_int),
+    ]
 
 
 class FmsConfig(ctypes.Structure):
@@ -90,7 +103,7 @@ EXPORTED_SYMBOLS = [
     "fms_enable_spike_detection", "fms_get_spikes", "fms_get_stimulation_log", "fms_schedule",
     "fms_last_error", "fms_set_tuning", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
-    "fms_reference_set_dims", "initializeArrays", "computeMonodomain",
+    "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -138,6 +151,8 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_set_sensors.argtypes = [v, _ip, i]
     lib.fms_adjoint_forward.argtypes = [v, _dp, i, i, i, _dp]
     lib.fms_adjoint_backward.argtypes = [v, _dp, _dp, _dp]
+    lib.fms_graph_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    lib.fms_autotune.argtypes = [v, _dp, i, i, i, ctypes.POINTER(FmsTuneResult), _dp, i]
     lib.fms_reference_set_dims.argtypes = [ctypes.POINTER(FmsReferenceDims)]
     lib.initializeArrays.argtypes = [_dp, _ip, _ip, ctypes.c_char_p, _dp, _ip, _ip, _dp, _dp, _dp, _dp]
     lib.initializeArrays.restype = None

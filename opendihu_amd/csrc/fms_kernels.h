@@ -159,10 +159,47 @@ __device__ __forceinline__ unsigned char stim_flag(const DevStim &s, int f, doub
   return (unsigned char)((stim ? FLAG_STIM : 0) | (compute ? FLAG_COMPUTE : 0));
 }
 
+// ----------------------------------------------------------------------------
+// The split-step kernels read their times from the device step table
+// (4 doubles per split step: startTime, dtSplit, dt0D, dt1D) through a cursor
+// in device memory: cursor[0] = row of the first step of the current launch
+// sequence, cursor[1] = the spike detector's step counter before row 0,
+// cursor[2] = the table's address. A captured graph of G steps therefore
+// replays over successive rows (k_cursor_advance) of whatever table the
+// current call uploaded.
+// ----------------------------------------------------------------------------
+struct StepRef {
+  const long long *cursor;
+  int q;   // step within the launch sequence
+  __device__ __forceinline__ const double *row() const {
+    return reinterpret_cast<const double *>(cursor[2]) + 4 * (cursor[0] + q);
+  }
+};
+
+// start time of a reaction half: the first half at startTime, the second at
+// startTime + dtSplit/2 (fast_monodomain_solver_compute.tpp:110-126)
+__device__ __forceinline__ double half_start(const double *row, int half) {
+  return half ? row[0] + 0.5 * row[1] : row[0];
+}
+
+__global__ void k_cursor_set(long long *cursor, long long row, long long step_counter, const double *table) {
+  if (threadIdx.x == 0) {
+    cursor[0] = row;
+    cursor[1] = step_counter;
+    cursor[2] = reinterpret_cast<long long>(table);
+  }
+}
+
+__global__ void k_cursor_advance(long long *cursor, int g) {
+  if (threadIdx.x == 0) cursor[0] += g;
+}
+
 // one thread per fibre: flags[f*n0D + j] for the n0D Heun steps of a half step
-__global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, double t0, double dt, int n0D) {
+__global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, StepRef sr, int half, int n0D) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n_fibers) return;
+  const double *row = sr.row();
+  const double t0 = half_start(row, half), dt = row[2];
   for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, t0 + j * dt);
 }
 
@@ -172,7 +209,7 @@ __global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, dou
 template <class M, class G, class E>
 __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, double *__restrict__ vm,
                                                   G *__restrict__ gates, const unsigned char *__restrict__ flags,
-                                                  double t0, double dt, int n0D, double vstim,
+                                                  StepRef sr, int half, int n0D, double vstim,
                                                   double *__restrict__ alg_transfer, const int *__restrict__ alg_idx,
                                                   int n_alg_transfer) {
   const long long N = fb.n_total;
@@ -180,6 +217,8 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   if (i >= N) return;
   const int f = fb.node_fiber[i];
   const int li = (int)(i - fb.fiber_off[f]);
+  const double *row = sr.row();
+  const double t0 = half_start(row, half), dt = row[2];
   double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
   for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
@@ -286,10 +325,11 @@ FMS_HD void thomas_serial_fiber(double *u, int n, const double *h, double prefac
 }
 
 // one lane per fibre: bit-exact reference recurrence
-__global__ __launch_bounds__(64) void k_thomas_serial(DevFibers fb, double *vm, double *cA, double *dA, double dt,
+__global__ __launch_bounds__(64) void k_thomas_serial(DevFibers fb, double *vm, double *cA, double *dA, StepRef sr,
                                                       int ie) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= fb.n_fibers) return;
+  const double dt = sr.row()[3];
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
   thomas_serial_fiber(vm + off, n, fb.elem_len + off - f, fb.prefactor[f], dt, ie != 0, cA + off, dA + off);
@@ -409,9 +449,10 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
 
 // one workgroup per fibre: load Vm to LDS, partitioned solve, store
 template <int T, int MMAX>
-__global__ __launch_bounds__(T) void k_thomas_part(DevFibers fb, double *vm, double dt, int ie) {
+__global__ __launch_bounds__(T) void k_thomas_part(DevFibers fb, double *vm, StepRef sr, int ie) {
   extern __shared__ double sh[];
   const int f = blockIdx.x;
+  const double dt = sr.row()[3];
   const int tid = threadIdx.x;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
@@ -425,9 +466,10 @@ __global__ __launch_bounds__(T) void k_thomas_part(DevFibers fb, double *vm, dou
 
 // spike detector: upward crossings of the threshold after each split step
 __global__ void k_spikes(long long N, const double *vm, double *vm_prev, int *first_step, int *count, double th,
-                         int step) {
+                         StepRef sr) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const int step = (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1);
   const double v = vm[i];
   const double p = vm_prev[i];
   if (p < th && v >= th) {

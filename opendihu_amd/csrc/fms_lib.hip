@@ -107,9 +107,18 @@ struct FmsSolver {
   int *d_sfirst = nullptr, *d_scount = nullptr;
   long long step_counter = 0;
 
-  // schedule knobs (fms_set_tuning): steps per fused launch, gate residency
+  // schedule knobs (fms_set_tuning): steps per fused launch, gate residency,
+  // threads per fibre workgroup, split steps per captured graph (0: no graph)
   int tune_chunk = 512;
   int tune_resident = -1;   // -1 auto, 0 stream gates from HBM, 1 keep gates in registers
+  int tune_threads = 256;
+  int tune_graph = 0;
+
+  // step cursor of the split kernels and the captured graph of tune_graph split steps
+  long long *d_cursor = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_sig;
+  long long n_graph_captures = 0, n_graph_launches = 0;
 
   // timing: events on the solver's stream and launch counters
   hipEvent_t events[8] = {};
@@ -134,6 +143,7 @@ struct FmsSolver {
 
   fms::DevFibers fibers() const {
     fms::DevFibers f;
+    memset(&f, 0, sizeof(f));
     f.n_fibers = F;
     f.n_total = N;
     f.fiber_off = d_off;
@@ -187,7 +197,8 @@ void free_all(FmsSolver *s) {
                   s->d_curjit, s->d_curr, s->d_hasbeen, s->d_log_count, s->d_log_fiber, s->d_log_time,
                   s->d_onset, s->d_dur, s->d_amp, s->d_vprev, s->d_sfirst, s->d_scount, s->d_alg_idx,
                   s->d_alg, s->d_sensor_nodes, s->d_sensor_slot, s->d_trace, s->d_dtrace, s->d_ckpt, s->d_tape,
-                  s->d_lam, s->d_gacc, s->d_pfbar, s->d_grad};
+                  s->d_lam, s->d_gacc, s->d_pfbar, s->d_grad, s->d_cursor};
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->events)
@@ -199,71 +210,177 @@ void free_all(FmsSolver *s) {
 // launchers, templated over gate type and exp policy
 // ---------------------------------------------------------------------------
 
+fms::StepRef step_ref(const FmsSolver *s, int q) { return fms::StepRef{s->d_cursor, q}; }
+
+// device step table: the K records of this call plus a copy of the last one
+// (the fused kernel reads one record ahead); the cursor starts at row 0 with
+// the current spike-step counter
+int upload_steps(FmsSolver *s, const double *steps, int K) {
+  if (s->steps_cap < 4 * (K + 1)) {
+    if (s->d_steps) HIP_TRY(hipFree(s->d_steps));
+    s->d_steps = nullptr;
+    s->steps_cap = 4 * (K + 1);
+    if (dalloc(&s->d_steps, (size_t)s->steps_cap)) return FMS_ERR_HIP;
+  }
+  std::vector<double> tab(4 * (size_t)(K + 1));
+  memcpy(tab.data(), steps, sizeof(double) * 4 * K);
+  for (int q = 0; q < 4; q++) tab[4 * (size_t)K + q] = steps[4 * (size_t)(K - 1) + q];
+  HIP_TRY(hipMemcpyAsync(s->d_steps, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(fms::k_cursor_set, dim3(1), dim3(64), 0, s->stream, s->d_cursor, 0LL, s->step_counter,
+                     (const double *)s->d_steps);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 template <class G, class E>
-int launch_reaction(FmsSolver *s, double t0, double dt, int n0D, bool store_alg) {
+int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
   hipLaunchKernelGGL((fms::k_reaction<Model, G, E>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
-                     s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, t0, dt, n0D, s->vstim,
-                     store_alg ? s->d_alg : nullptr, s->d_alg_idx, s->n_alg_idx);
+                     s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+                     s->vstim, store_alg ? s->d_alg : nullptr, s->d_alg_idx, s->n_alg_idx);
   HIP_TRY(hipGetLastError());
   s->n_launch_reaction++;
   return 0;
 }
 
-int launch_flags(FmsSolver *s, double t0, double dt, int n0D) {
+int launch_flags(FmsSolver *s, int q, int half, int n0D) {
   const int threads = 64;
   hipLaunchKernelGGL(fms::k_stim_flags, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
-                     s->stim(), s->F, s->d_flags, t0, dt, n0D);
+                     s->stim(), s->F, s->d_flags, step_ref(s, q), half, n0D);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
-int launch_thomas(FmsSolver *s, double dt1) {
+int launch_thomas(FmsSolver *s, int q) {
   const int ie = s->scheme == FMS_IMPLICIT_EULER;
   if (s->thomas == FMS_THOMAS_SERIAL) {
     const int threads = 64;
     hipLaunchKernelGGL(fms::k_thomas_serial, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
-                       s->fibers(), s->d_vm, s->d_cA, s->d_dA, dt1, ie);
+                       s->fibers(), s->d_vm, s->d_cA, s->d_dA, step_ref(s, q), ie);
   } else {
     const size_t lds = (size_t)(16 * PART_T + s->nmax) * sizeof(double);
     hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 16>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
-                       s->d_vm, dt1, ie);
+                       s->d_vm, step_ref(s, q), ie);
   }
   HIP_TRY(hipGetLastError());
   s->n_launch_thomas++;
   return 0;
 }
 
-int launch_spikes(FmsSolver *s) {
+int launch_spikes(FmsSolver *s, int q) {
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
   hipLaunchKernelGGL(fms::k_spikes, dim3((unsigned)blocks), dim3(threads), 0, s->stream, s->N, s->d_vm,
-                     s->d_vprev, s->d_sfirst, s->d_scount, s->spike_th, (int)s->step_counter);
+                     s->d_vprev, s->d_sfirst, s->d_scount, s->spike_th, step_ref(s, q));
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
+// one Strang step on the split kernels: reaction half, diffusion, reaction half
+// (fast_monodomain_solver_compute.tpp:110-126), step q of the launch sequence
 template <class G, class E>
-int run_split(FmsSolver *s, const double *steps, int K, int n0D) {
+int enqueue_split_step(FmsSolver *s, int q, int n0D, bool store_alg) {
   int rc;
-  for (int k = 0; k < K; k++) {
-    const double t = steps[4 * k], dts = steps[4 * k + 1], dt0 = steps[4 * k + 2], dt1 = steps[4 * k + 3];
-    const double tmid = t + 0.5 * dts;
-    if ((rc = launch_flags(s, t, dt0, n0D))) return rc;
-    if ((rc = launch_reaction<G, E>(s, t, dt0, n0D, false))) return rc;
-    if ((rc = launch_thomas(s, dt1))) return rc;
-    if ((rc = launch_flags(s, tmid, dt0, n0D))) return rc;
-    if ((rc = launch_reaction<G, E>(s, tmid, dt0, n0D, s->d_alg && k == K - 1))) return rc;
-    s->step_counter++;
-    if (s->spikes && (rc = launch_spikes(s))) return rc;
-  }
+  if ((rc = launch_flags(s, q, 0, n0D))) return rc;
+  if ((rc = launch_reaction<G, E>(s, q, 0, n0D, false))) return rc;
+  if ((rc = launch_thomas(s, q))) return rc;
+  if ((rc = launch_flags(s, q, 1, n0D))) return rc;
+  if ((rc = launch_reaction<G, E>(s, q, 1, n0D, store_alg))) return rc;
+  if (s->spikes && (rc = launch_spikes(s, q))) return rc;
   return 0;
 }
 
-template <class G, class E, int NPT, int MMAX>
+// everything a captured graph of split steps has baked into its kernel arguments
+template <class G, class E>
+std::vector<unsigned char> graph_signature(const FmsSolver *s, int gsteps, int n0D) {
+  const fms::DevFibers fb = s->fibers();
+  const fms::DevStim st = s->stim();
+  struct {
+    const void *vm, *gates, *flags, *cursor, *cA, *dA, *vprev, *sfirst, *scount;
+    double vstim, spike_th;
+    int spikes, n0D, gsteps, thomas, scheme, nmax, g_is_double, e_is_approx;
+  } k;
+  memset(&k, 0, sizeof(k));
+  k.vm = s->d_vm; k.gates = s->d_gates; k.flags = s->d_flags; k.cursor = s->d_cursor;
+  k.cA = s->d_cA; k.dA = s->d_dA; k.vprev = s->d_vprev; k.sfirst = s->d_sfirst; k.scount = s->d_scount;
+  k.vstim = s->vstim; k.spike_th = s->spike_th; k.spikes = s->spikes; k.n0D = n0D; k.gsteps = gsteps;
+  k.thomas = s->thomas; k.scheme = s->scheme; k.nmax = s->nmax;
+  k.g_is_double = std::is_same<G, double>::value;
+  k.e_is_approx = std::is_same<E, fms::ExpApprox>::value;
+  std::vector<unsigned char> sig(sizeof(fb) + sizeof(st) + sizeof(k));
+  memcpy(sig.data(), &fb, sizeof(fb));
+  memcpy(sig.data() + sizeof(fb), &st, sizeof(st));
+  memcpy(sig.data() + sizeof(fb) + sizeof(st), &k, sizeof(k));
+  return sig;
+}
+
+void drop_graph(FmsSolver *s) {
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  s->graph_exec = nullptr;
+  s->graph_sig.clear();
+}
+
+// capture gsteps split steps and the cursor advance as one graph
+template <class G, class E>
+int capture_split_graph(FmsSolver *s, int gsteps, int n0D) {
+  drop_graph(s);
+  const long long nr = s->n_launch_reaction, nt = s->n_launch_thomas;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int q = 0; q < gsteps && !rc; q++) rc = enqueue_split_step<G, E>(s, q, n0D, false);
+  if (!rc) {
+    hipLaunchKernelGGL(fms::k_cursor_advance, dim3(1), dim3(64), 0, s->stream, s->d_cursor, gsteps);
+    if (hipGetLastError() != hipSuccess) rc = fail(FMS_ERR_HIP, "cursor kernel launch failed during capture");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  s->n_launch_reaction = nr;
+  s->n_launch_thomas = nt;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FMS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    s->graph_exec = nullptr;
+    return fail(FMS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  s->graph_sig = graph_signature<G, E>(s, gsteps, n0D);
+  s->n_graph_captures++;
+  return 0;
+}
+
+// split kernels: K steps, whole groups of tune_graph steps replayed as one
+// captured graph, the rest (and a step that returns algebraics) launched eagerly
+template <class G, class E>
+int run_split(FmsSolver *s, const double *h_steps, int K, int n0D) {
+  int rc;
+  if ((rc = upload_steps(s, h_steps, K))) return rc;
+  const int gs = s->tune_graph;
+  const int graph_steps_max = s->d_alg ? K - 1 : K;
+  int done = 0;
+  if (gs > 0 && graph_steps_max >= gs) {
+    if (s->graph_sig != graph_signature<G, E>(s, gs, n0D) || !s->graph_exec)
+      if ((rc = capture_split_graph<G, E>(s, gs, n0D))) return rc;
+    const int nrep = graph_steps_max / gs;
+    for (int r = 0; r < nrep; r++) HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+    done = nrep * gs;
+    s->n_launch_reaction += 2LL * done;
+    s->n_launch_thomas += done;
+    s->n_graph_launches += nrep;
+  }
+  // the cursor now points at row `done`
+  for (int q = 0; done + q < K; q++)
+    if ((rc = enqueue_split_step<G, E>(s, q, n0D, s->d_alg && done + q == K - 1))) return rc;
+  s->step_counter += K;
+  return 0;
+}
+
+template <class G, class E, int T, int NPT, int MMAX>
 int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr) {
-  constexpr int T = FUSED_T;
   fms::DevSpikes spk;
   spk.enabled = s->spikes;
   spk.threshold = s->spike_th;
@@ -282,20 +399,29 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const 
   return 0;
 }
 
-// gates resident in registers when the whole fibre fits the workgroup's
-// registers (small models), streamed from HBM otherwise; the partitioned
-// solve needs ceil(n / min(T, n/2)) <= MMAX rows per thread
+// Fused schedules: one workgroup of T threads per fibre; gates resident in
+// registers (NPT nodes per thread, small models) or streamed from HBM. The
+// partitioned solve needs ceil(n / T) <= MMAX rows per thread, the serial
+// solve 2n <= 16 T doubles of LDS scratch (fms_set_tuning checks both).
 template <class G, class E>
 int launch_fused(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr = fms::DevTrace{}) {
+  const bool resident = s->tune_resident != 0;
+  if constexpr (Model::NS <= 8) {
+    if (s->tune_threads == 128) {
+      const int npt = (s->nmax + 127) / 128;
+      if (resident && npt <= 8) return launch_fused_npt<G, E, 128, 8, 16>(s, d_steps, K, n0D, tr);
+      if (resident && npt <= 16) return launch_fused_npt<G, E, 128, 16, 16>(s, d_steps, K, n0D, tr);
+      return launch_fused_npt<G, E, 128, 0, 16>(s, d_steps, K, n0D, tr);
+    }
+  }
   constexpr int T = FUSED_T;
   const int npt = (s->nmax + T - 1) / T;
   if constexpr (Model::NS <= 8) {
-    const bool resident = s->tune_resident != 0;
-    if (resident && npt <= 4) return launch_fused_npt<G, E, 4, 8>(s, d_steps, K, n0D, tr);
-    if (resident && npt <= 8) return launch_fused_npt<G, E, 8, 8>(s, d_steps, K, n0D, tr);
+    if (resident && npt <= 4) return launch_fused_npt<G, E, T, 4, 8>(s, d_steps, K, n0D, tr);
+    if (resident && npt <= 8) return launch_fused_npt<G, E, T, 8, 8>(s, d_steps, K, n0D, tr);
   }
-  if (npt <= 8) return launch_fused_npt<G, E, 0, 8>(s, d_steps, K, n0D, tr);
-  return launch_fused_npt<G, E, 0, 16>(s, d_steps, K, n0D, tr);
+  if (npt <= 8) return launch_fused_npt<G, E, T, 0, 8>(s, d_steps, K, n0D, tr);
+  return launch_fused_npt<G, E, T, 0, 16>(s, d_steps, K, n0D, tr);
 }
 
 template <class G, class E>
@@ -321,20 +447,11 @@ int run(FmsSolver *s, const double *h_steps, int K, int n0D) {
 template <class G, class E>
 int run(FmsSolver *s, const double *h_steps, int K, int n0D, bool fused_only) {
   (void)fused_only;
-  // fused: upload the step table (4 doubles + one look-ahead record) and launch in chunks
-  const int chunk = s->tune_chunk;
-  if (s->steps_cap < 4 * (K + 1)) {
-    if (s->d_steps) HIP_TRY(hipFree(s->d_steps));
-    s->steps_cap = 4 * (K + 1);
-    if (dalloc(&s->d_steps, (size_t)s->steps_cap)) return FMS_ERR_HIP;
-  }
-  std::vector<double> tab(4 * (size_t)(K + 1));
-  memcpy(tab.data(), h_steps, sizeof(double) * 4 * K);
-  for (int q = 0; q < 4; q++) tab[4 * (size_t)K + q] = h_steps[4 * (size_t)(K - 1) + q];
-  HIP_TRY(hipMemcpyAsync(s->d_steps, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  // the fused kernel reads step s+1's record for the first half of step s+1;
+  // fused: the kernel reads step s+1's record for the first half of step s+1;
   // a launch covers steps [k0, k0+kc) and then finishes the last step itself
   int rc;
+  if ((rc = upload_steps(s, h_steps, K))) return rc;
+  const int chunk = s->tune_chunk;
   for (int k0 = 0; k0 < K; k0 += chunk) {
     const int kc = std::min(chunk, K - k0);
     if ((rc = launch_fused<G, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D))) return rc;
@@ -438,6 +555,11 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
     delete s;
     return fail(FMS_ERR_UNSUPPORTED, "fibres longer than 4096 nodes need thomas=SERIAL and kernels=SPLIT");
   }
+  if (s->nmax > 2048 && s->thomas == FMS_THOMAS_SERIAL && s->kernels == FMS_KERNELS_FUSED) {
+    // the fused serial solve keeps its two n-long recurrences in the 16*256-double LDS scratch
+    delete s;
+    return fail(FMS_ERR_UNSUPPORTED, "the fused kernel's serial solve takes fibres of <= 2048 nodes");
+  }
   // fibres longest first for the fused kernel (LPT over the dispatcher)
   s->h_order.resize(s->F);
   for (int f = 0; f < s->F; f++) s->h_order[f] = f;
@@ -464,6 +586,7 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
   chk(dalloc(&s->d_params, (size_t)F * (Model::NP > 0 ? Model::NP : 1)));
   chk(dalloc(&s->d_vm, (size_t)N));
   chk(dalloc(&s->d_hasbeen, (size_t)F));
+  chk(dalloc(&s->d_cursor, 4));
   chk(hipMalloc(&s->d_gates, (size_t)(Model::NS - 1) * N * gate_bytes(s) + 8) == hipSuccess ? 0 : FMS_ERR_HIP);
   if (s->thomas == FMS_THOMAS_SERIAL && s->kernels == FMS_KERNELS_SPLIT) {
     chk(dalloc(&s->d_cA, (size_t)N));
@@ -752,9 +875,23 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       return 0;
     case FMS_TUNE_RESIDENT_GATES:
       if (value < -1 || value > 1) return fail(FMS_ERR_ARGUMENT, "resident must be -1, 0 or 1");
-      if (value == 1 && ((s->nmax + FUSED_T - 1) / FUSED_T > 8 || Model::NS > 8))
+      if (value == 1 && (s->nmax > 2048 || Model::NS > 8))
         return fail(FMS_ERR_UNSUPPORTED, "resident gates need fibres of <= 2048 nodes and <= 8 states");
       s->tune_resident = value;
+      return 0;
+    case FMS_TUNE_THREADS:
+      if (value != 128 && value != 256) return fail(FMS_ERR_ARGUMENT, "threads per fibre must be 128 or 256");
+      if (value == 128) {
+        if (Model::NS > 8) return fail(FMS_ERR_UNSUPPORTED, "128 threads per fibre are built for models of <= 8 states");
+        if (s->nmax > 2048) return fail(FMS_ERR_UNSUPPORTED, "128 threads per fibre need fibres of <= 2048 nodes");
+        if (s->thomas == FMS_THOMAS_SERIAL && s->nmax > 1024)
+          return fail(FMS_ERR_UNSUPPORTED, "128 threads per fibre with the serial solve need fibres of <= 1024 nodes");
+      }
+      s->tune_threads = value;
+      return 0;
+    case FMS_TUNE_GRAPH_STEPS:
+      if (value < 0 || value > 4096) return fail(FMS_ERR_ARGUMENT, "graph steps must be 0..4096");
+      s->tune_graph = value;
       return 0;
     default:
       return fail(FMS_ERR_ARGUMENT, "unknown tuning key");
@@ -775,6 +912,13 @@ int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
   float f = 0.f;
   HIP_TRY(hipEventElapsedTime(&f, s->events[from], s->events[to]));
   *ms = f;
+  return 0;
+}
+
+int fms_graph_counts(FmsSolver *s, long long *captures, long long *replays) {
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  if (captures) *captures = s->n_graph_captures;
+  if (replays) *replays = s->n_graph_launches;
   return 0;
 }
 
@@ -803,20 +947,6 @@ template <class T> int ensure(T **p, long long &cap, long long n) {
   cap = 0;
   if (dalloc(p, (size_t)n)) return FMS_ERR_HIP;
   cap = n;
-  return 0;
-}
-
-int upload_steps(FmsSolver *s, const double *steps, int K) {
-  if (s->steps_cap < 4 * (K + 1)) {
-    if (s->d_steps) HIP_TRY(hipFree(s->d_steps));
-    s->d_steps = nullptr;
-    s->steps_cap = 4 * (K + 1);
-    if (dalloc(&s->d_steps, (size_t)s->steps_cap)) return FMS_ERR_HIP;
-  }
-  std::vector<double> tab(4 * (size_t)(K + 1));
-  memcpy(tab.data(), steps, sizeof(double) * 4 * K);
-  for (int q = 0; q < 4; q++) tab[4 * (size_t)K + q] = steps[4 * (size_t)(K - 1) + q];
-  HIP_TRY(hipMemcpyAsync(s->d_steps, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
   return 0;
 }
 
@@ -987,6 +1117,177 @@ int fms_adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_param
   if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
   if (s->exp_mode == FMS_EXP_APPROXIMATE) return adjoint_backward<fms::ExpApprox>(s, dtraces, grad_params, grad_prefactor);
   return adjoint_backward<fms::ExpLibm>(s, dtraces, grad_params, grad_prefactor);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// autotuner: times the schedules that give bit-identical results on the
+// solver's own batch and keeps the fastest
+// ===========================================================================
+namespace {
+
+// device state that a trial run changes: states, the stimulation state
+// machine, the stimulation log counter and the spike detector
+struct Snapshot {
+  std::vector<std::pair<void *, size_t>> regions;
+  std::vector<void *> copies;
+  long long step_counter = 0, n_fused = 0, n_reaction = 0, n_thomas = 0, n_gcap = 0, n_glaunch = 0;
+};
+
+int snapshot_take(FmsSolver *s, Snapshot &sn) {
+  const size_t N = (size_t)s->N, F = (size_t)s->F;
+  auto add = [&](void *p, size_t bytes) {
+    if (p && bytes) sn.regions.push_back({p, bytes});
+  };
+  add(s->d_vm, N * sizeof(double));
+  add(s->d_gates, (size_t)(Model::NS - 1) * N * gate_bytes(s));
+  add(s->d_last, F * sizeof(double));
+  add(s->d_curjit, F * sizeof(double));
+  add(s->d_jidx, F * sizeof(int));
+  add(s->d_curr, F);
+  add(s->d_hasbeen, F);
+  add(s->d_log_count, sizeof(int));
+  if (s->spikes) {
+    add(s->d_vprev, N * sizeof(double));
+    add(s->d_sfirst, N * sizeof(int));
+    add(s->d_scount, N * sizeof(int));
+  }
+  for (auto &r : sn.regions) {
+    void *c = nullptr;
+    HIP_TRY(hipMalloc(&c, r.second));
+    sn.copies.push_back(c);
+    HIP_TRY(hipMemcpyAsync(c, r.first, r.second, hipMemcpyDeviceToDevice, s->stream));
+  }
+  sn.step_counter = s->step_counter;
+  sn.n_fused = s->n_launch_fused;
+  sn.n_reaction = s->n_launch_reaction;
+  sn.n_thomas = s->n_launch_thomas;
+  sn.n_gcap = s->n_graph_captures;
+  sn.n_glaunch = s->n_graph_launches;
+  return 0;
+}
+
+int snapshot_restore(FmsSolver *s, const Snapshot &sn) {
+  for (size_t k = 0; k < sn.regions.size(); k++)
+    HIP_TRY(hipMemcpyAsync(sn.regions[k].first, sn.copies[k], sn.regions[k].second, hipMemcpyDeviceToDevice,
+                           s->stream));
+  s->step_counter = sn.step_counter;
+  s->n_launch_fused = sn.n_fused;
+  s->n_launch_reaction = sn.n_reaction;
+  s->n_launch_thomas = sn.n_thomas;
+  s->n_graph_captures = sn.n_gcap;
+  s->n_graph_launches = sn.n_glaunch;
+  return 0;
+}
+
+void snapshot_free(Snapshot &sn) {
+  for (void *c : sn.copies)
+    if (c) (void)hipFree(c);
+  sn.copies.clear();
+}
+
+struct Schedule {
+  int kernels, chunk, resident, threads, graph;
+};
+
+int apply_schedule(FmsSolver *s, const Schedule &c) {
+  s->kernels = c.kernels;
+  s->tune_chunk = c.chunk;
+  s->tune_resident = c.resident;
+  s->tune_threads = c.threads;
+  s->tune_graph = c.graph;
+  if (c.kernels == FMS_KERNELS_SPLIT && s->thomas == FMS_THOMAS_SERIAL && !s->d_cA) {
+    if (dalloc(&s->d_cA, (size_t)s->N) || dalloc(&s->d_dA, (size_t)s->N)) return FMS_ERR_HIP;
+  }
+  return 0;
+}
+
+// candidates in the order they are tried; every one computes the same
+// arithmetic in the same order (bit-identical states)
+std::vector<Schedule> candidates(const FmsSolver *s, int allow_split) {
+  std::vector<Schedule> v;
+  const bool small = Model::NS <= 8 && s->nmax <= 2048;
+  const bool fused_ok = !(s->nmax > 2048 && s->thomas == FMS_THOMAS_SERIAL) && s->nmax <= 4096;
+  if (fused_ok) {
+    for (int chunk : {64, 512}) {
+      if (small) {
+        v.push_back({FMS_KERNELS_FUSED, chunk, 1, 256, 0});
+        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0});
+        if (!(s->thomas == FMS_THOMAS_SERIAL && s->nmax > 1024)) {
+          v.push_back({FMS_KERNELS_FUSED, chunk, 1, 128, 0});
+          v.push_back({FMS_KERNELS_FUSED, chunk, 0, 128, 0});
+        }
+      } else {
+        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0});
+      }
+    }
+  }
+  if (allow_split || !fused_ok)
+    for (int g : {0, 8, 32}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g});
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int allow_split, FmsTuneResult *result,
+                 double *candidate_ms, int capacity) {
+  if (!s || !steps || n_steps < 1 || n0D < 1) return fail(FMS_ERR_ARGUMENT, "bad autotune arguments");
+  if (s->d_alg) return fail(FMS_ERR_STATE, "autotune before requesting algebraics for transfer");
+  const std::vector<Schedule> cand = candidates(s, allow_split);
+  if (cand.empty()) return fail(FMS_ERR_UNSUPPORTED, "no schedule fits this batch");
+  const Schedule keep{s->kernels, s->tune_chunk, s->tune_resident, s->tune_threads, s->tune_graph};
+  Snapshot sn;
+  int rc = snapshot_take(s, sn);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(FMS_ERR_HIP, "hipEventCreate failed");
+  int best = -1;
+  double best_ms = 0;
+  for (size_t k = 0; k < cand.size() && !rc; k++) {
+    if ((rc = apply_schedule(s, cand[k]))) break;
+    // one untimed pass (graph capture, code object load), then the timed pass
+    for (int pass = 0; pass < 2 && !rc; pass++) {
+      if ((rc = snapshot_restore(s, sn))) break;
+      if (pass == 1 && hipEventRecord(e0, s->stream) != hipSuccess) rc = fail(FMS_ERR_HIP, "hipEventRecord");
+      if (!rc) rc = dispatch(s, steps, n_steps, n0D);
+      if (!rc && pass == 1 && hipEventRecord(e1, s->stream) != hipSuccess) rc = fail(FMS_ERR_HIP, "hipEventRecord");
+    }
+    if (rc) break;
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(FMS_ERR_HIP, "hipEventSynchronize");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (candidate_ms && (int)k < capacity) candidate_ms[k] = ms / n_steps;
+    if (best < 0 || ms < best_ms) {
+      best = (int)k;
+      best_ms = ms;
+    }
+  }
+  if (!rc) rc = snapshot_restore(s, sn);
+  if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FMS_ERR_HIP, "hipStreamSynchronize");
+  snapshot_free(sn);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) {
+    (void)apply_schedule(s, keep);
+    return rc;
+  }
+  if ((rc = apply_schedule(s, cand[best]))) return rc;
+  if (result) {
+    result->kernels = cand[best].kernels;
+    result->steps_per_launch = cand[best].chunk;
+    result->resident_gates = cand[best].resident;
+    result->threads = cand[best].threads;
+    result->graph_steps = cand[best].graph;
+    result->ms_per_step = best_ms / n_steps;
+    result->n_candidates = (int)cand.size();
+  }
+  return 0;
 }
 
 }  // extern "C"

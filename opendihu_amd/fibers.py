@@ -156,14 +156,34 @@ class FiberBatch:
         self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_ONLY_IF_STIMULATED, int(bool(on))),
                   "fms_set_tuning")
 
-    def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None) -> None:
-        """fused-kernel schedule: split steps per launch; gates in registers (1), streamed (0) or auto (-1)"""
-        if steps_per_launch is not None:
-            self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_STEPS_PER_LAUNCH, int(steps_per_launch)),
-                      "fms_set_tuning")
-        if resident_gates is not None:
-            self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_RESIDENT_GATES, int(resident_gates)),
-                      "fms_set_tuning")
+    def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None,
+                   threads: int | None = None, graph_steps: int | None = None) -> None:
+        """Schedule knobs. Fused kernel: split steps per launch; gates in registers (1),
+        streamed (0) or auto (-1); threads per fibre workgroup (128 or 256). Split
+        kernels: steps per captured hipGraph (0 launches eagerly)."""
+        for key, value in ((L.FMS_TUNE_STEPS_PER_LAUNCH, steps_per_launch), (L.FMS_TUNE_RESIDENT_GATES, resident_gates),
+                           (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps)):
+            if value is not None:
+                self._chk(self.lib.fms_set_tuning(self.handle, key, int(value)), "fms_set_tuning")
+
+    def graph_counts(self) -> dict:
+        c, r = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self.lib.fms_graph_counts(self.handle, ctypes.byref(c), ctypes.byref(r)), "fms_graph_counts")
+        return dict(captures=c.value, replays=r.value)
+
+    def autotune(self, steps, n0D: int = 1, allow_split: bool = True) -> dict:
+        """Time every bit-identical schedule over the step records `steps` (n, 4)
+        on this batch's own states, restore the states, keep the fastest."""
+        steps = np.ascontiguousarray(steps, dtype=np.float64).reshape(-1, 4)
+        res = L.FmsTuneResult()
+        ms = np.full(64, np.nan)
+        self._chk(self.lib.fms_autotune(self.handle, L.as_ptr(steps, L._dp), steps.shape[0], int(n0D),
+                                        int(bool(allow_split)), ctypes.byref(res), L.as_ptr(ms, L._dp), ms.size),
+                  "fms_autotune")
+        self.config["kernels"] = "fused" if res.kernels == L.FMS_KERNELS_FUSED else "split"
+        return dict(kernels=self.config["kernels"], steps_per_launch=res.steps_per_launch,
+                    resident_gates=res.resident_gates, threads=res.threads, graph_steps=res.graph_steps,
+                    ms_per_step=res.ms_per_step, candidates_ms=ms[:res.n_candidates].tolist())
 
     def record_event(self, slot: int) -> None:
         self._chk(self.lib.fms_record_event(self.handle, int(slot)), "fms_record_event")

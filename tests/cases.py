@@ -56,19 +56,22 @@ def firing_table(n_rows, n_mu, rate_per_ms, dt_row, seed):
     return (rng.random((n_rows, n_mu)) < rate_per_ms * dt_row).astype(np.uint8)
 
 
-def poisson_firing_table(n_rows, n_mu, rate_per_ms, dt_row, duration_ms, seed):
+def poisson_firing_table(n_rows, n_mu, rate_per_ms, dt_row, duration_ms, seed, first_at_zero=False):
     """Firing table with one row per sampling step: each motor unit fires at
-    Poisson times (rate_per_ms) and each firing marks duration_ms of rows."""
+    Poisson times (rate_per_ms, a scalar or one rate per motor unit) and each
+    firing marks duration_ms of rows; first_at_zero adds a firing of every unit
+    at t=0."""
     rng = np.random.default_rng(seed)
+    rates = np.broadcast_to(np.asarray(rate_per_ms, np.float64), (n_mu,))
     table = np.zeros((n_rows, n_mu), np.uint8)
     t_end = n_rows * dt_row
     width = max(1, int(round(duration_ms / dt_row)))
     for mu in range(n_mu):
-        t = rng.exponential(1.0 / rate_per_ms)
+        t = 0.0 if first_at_zero else rng.exponential(1.0 / rates[mu])
         while t < t_end:
             r0 = int(t / dt_row)
             table[r0:r0 + width, mu] = 1
-            t += rng.exponential(1.0 / rate_per_ms)
+            t += rng.exponential(1.0 / rates[mu])
     return table
 
 
@@ -85,6 +88,26 @@ def c2(n_fibers=1024, n_nodes=1481, length_cm=14.8, dt=2.5e-5, end_time=5.0, n_m
     table = poisson_firing_table(rows, n_mu, rate_hz * 1e-3, dt, 0.1, seed_fire)
     off = np.arange(n_fibers, dtype=np.int64) * n_nodes
     sb, se = vc_lane_stim_range(off, np.full(n_fibers, n_nodes // 2), n, lane_width)
+    stim = dict(stim_begin=sb, stim_end=se, firing_events=table, motor_unit=mu, call_frequency=1.0 / dt,
+                repeat_after_first_call=0.0, call_enable_begin=0.0)
+    return dict(n_nodes=n, element_lengths=h, prefactor=pf), stim
+
+
+def c5(n_fibers=16384, lo=1000, hi=1481, n_mu=100, rate_hz=(10.0, 40.0), dt=2.5e-5, n_rows=4000, seed=5,
+       lane_width=4, first_at_zero=True):
+    """Large Shorten-type batch: fibres of U{lo..hi} nodes (h = 0.01 cm), motor
+    units drawn uniformly, each firing at seeded Poisson times with its own rate
+    ~ U(rate_hz) at the fibre's centre node; the firing table has one row per
+    split step (n_rows rows, wrapping as the reference's table does)."""
+    rng = np.random.default_rng(seed)
+    n = rng.integers(lo, hi + 1, size=n_fibers).astype(np.int32)
+    h = np.full(int(n.sum()) - n_fibers, 0.01)
+    pf = np.full(n_fibers, SIGMA / (AM * CM))
+    mu = np.random.default_rng(seed + 1).integers(1, n_mu + 1, size=n_fibers).astype(np.int32)
+    rates = np.random.default_rng(seed + 2).uniform(rate_hz[0], rate_hz[1], size=n_mu) * 1e-3
+    table = poisson_firing_table(n_rows, n_mu, rates, dt, 0.1, seed + 3, first_at_zero=first_at_zero)
+    off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    sb, se = vc_lane_stim_range(off, n // 2, n, lane_width)
     stim = dict(stim_begin=sb, stim_end=se, firing_events=table, motor_unit=mu, call_frequency=1.0 / dt,
                 repeat_after_first_call=0.0, call_enable_begin=0.0)
     return dict(n_nodes=n, element_lengths=h, prefactor=pf), stim

@@ -1,0 +1,123 @@
+"""Schedules that must not change a single bit: split steps replayed as captured
+hipGraphs, the fused kernel with 128 or 256 threads per fibre, and the
+autotuner (which runs trial steps on the batch and must leave its states,
+stimulation state machine, log and spike counters as they were).
+
+Each graph test replays the graph many times across several advance calls with
+host reads in between and checks every call against the eagerly launched run.
+"""
+import numpy as np
+import pytest
+
+from cases import uneven, vc_lane_stim_range, firing_table
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(mesh, **kw):
+    from opendihu_amd.fibers import FiberBatch
+    return FiberBatch(**mesh, **kw)
+
+
+def _firing(batch, mesh):
+    n = mesh["n_nodes"]
+    F = n.size
+    off = np.concatenate([[0], np.cumsum(n)[:-1]])
+    sb, se = vc_lane_stim_range(off, n // 2, n, 4)
+    fe = firing_table(40, 10, 4.0, 0.1, seed=2)
+    mu = (np.arange(F) * 7) % 10 + 1
+    jit = [[0.1, -0.1, 0.05][: (f % 4)] for f in range(F)]
+    batch.set_firing_table_stimulation(sb, se, fe, mu, call_frequency=1.0, repeat_after_first_call=0.1,
+                                       call_enable_begin=0.2 * (np.arange(F) % 3), jitter_lists=jit,
+                                       log_capacity=8192)
+
+
+@pytest.mark.parametrize("thomas", ["serial", "partitioned"])
+def test_graph_replay_equals_eager(thomas):
+    mesh, _ = uneven(12, lo=200, hi=600, seed=9)
+    runs = {}
+    for mode in ("eager", "graph"):
+        g = _batch(mesh, scheme="IE", kernels="split", thomas=thomas)
+        if mode == "graph":
+            g.set_tuning(graph_steps=8)
+        _firing(g, mesh)
+        g.enable_spike_detection(0.0)
+        t, dt, per_call = 0.0, 2e-3, []
+        for k in (37, 8, 50, 3, 400, 1002):
+            g.advance(t, dt, k)
+            t += k * dt
+            per_call.append(g.get_states())
+        runs[mode] = (per_call, g.spikes(), g.stimulation_log(), g.graph_counts())
+    for c, (a, b) in enumerate(zip(runs["eager"][0], runs["graph"][0])):
+        np.testing.assert_array_equal(a, b, err_msg=f"advance call {c}")
+    for a, b in zip(runs["eager"][1], runs["graph"][1]):
+        np.testing.assert_array_equal(a, b)
+    ga = sorted(zip(runs["graph"][2][1].tolist(), runs["graph"][2][0].tolist()))
+    ea = sorted(zip(runs["eager"][2][1].tolist(), runs["eager"][2][0].tolist()))
+    assert ga == ea and len(ea) > 0
+    counts = runs["graph"][3]
+    print("graph counts", counts)
+    assert counts["captures"] == 1, "the captured graph is reused across advance calls"
+    assert counts["replays"] == sum(k // 8 for k in (37, 8, 50, 3, 400, 1002))
+    assert runs["eager"][3]["replays"] == 0
+
+
+def test_graph_recaptured_after_stimulation_change():
+    mesh, stim = uneven(6, lo=300, hi=500, seed=3)
+    out = []
+    for gs in (0, 16):
+        g = _batch(mesh, scheme="CN", kernels="split", thomas="partitioned")
+        g.set_tuning(graph_steps=gs)
+        g.set_current_stimulation(**stim)
+        g.advance(0.0, 1e-3, 64)
+        st2 = dict(stim, onset=stim["onset"] + 0.5, amplitude=stim["amplitude"] * 0.8)
+        g.set_current_stimulation(**st2)
+        g.advance(0.064, 1e-3, 640)
+        out.append((g.get_states(), g.graph_counts()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[1][1]["captures"] == 2
+
+
+@pytest.mark.parametrize("thomas", ["serial", "partitioned"])
+@pytest.mark.parametrize("resident", [0, 1])
+def test_fused_128_threads_equals_256(thomas, resident):
+    mesh, stim = uneven(10, lo=300, hi=1000 if thomas == "serial" else 2000, seed=8)
+    runs = []
+    for threads in (256, 128):
+        g = _batch(mesh, scheme="CN", thomas=thomas, kernels="fused")
+        g.set_tuning(threads=threads, resident_gates=resident, steps_per_launch=77)
+        g.set_current_stimulation(**stim)
+        g.enable_spike_detection(0.0)
+        g.advance(0.0, 1e-3, 700)
+        runs.append((g.get_states(), g.spikes()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1][0], runs[1][1][0])
+
+
+def test_autotune_keeps_state_and_bits():
+    mesh, _ = uneven(16, lo=300, hi=900, seed=5)
+    dt = 2e-3
+    trial = np.stack([np.arange(40) * dt + 1.0, np.full(40, dt), np.full(40, dt), np.full(40, dt)], 1)
+    ref = _batch(mesh, scheme="IE", kernels="split", thomas="serial")
+    tuned = _batch(mesh, scheme="IE", kernels="split", thomas="serial")
+    for b in (ref, tuned):
+        _firing(b, mesh)
+        b.enable_spike_detection(0.0)
+        b.advance(0.0, dt, 300)
+    before = (tuned.get_states(), tuned.spikes(), tuned.stimulation_log())
+    choice = tuned.autotune(trial, allow_split=True)
+    print("autotune choice", choice)
+    after = (tuned.get_states(), tuned.spikes(), tuned.stimulation_log())
+    np.testing.assert_array_equal(before[0], after[0])
+    for a, b in zip(before[1], after[1]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(before[2], after[2]):
+        np.testing.assert_array_equal(a, b)
+    assert len(choice["candidates_ms"]) >= 3 and all(np.isfinite(choice["candidates_ms"]))
+    for b in (ref, tuned):
+        b.advance(300 * dt, dt, 700)
+    np.testing.assert_array_equal(ref.get_states(), tuned.get_states())
+    for a, b in zip(ref.spikes(), tuned.spikes()):
+        np.testing.assert_array_equal(a, b)
+    ra, ta = ref.stimulation_log(), tuned.stimulation_log()
+    assert sorted(zip(ra[1].tolist(), ra[0].tolist())) == sorted(zip(ta[1].tolist(), ta[0].tolist()))
