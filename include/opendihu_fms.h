@@ -102,6 +102,19 @@ int fms_model_defaults(double *states /* [n_states] */, double *params /* [n_par
    each exp/log counts one; integer powers count their multiplications) */
 int fms_model_ops(double *ops_rhs, double *divs_rhs, double *exps_rhs);
 
+/* The model's right-hand side evaluated on the device for n independent
+   points (the generated computeRates of the reference's CellML model source,
+   GEN/03_generator_vc.cpp; CellmlAdapter evaluateTimesteppingRightHandSide):
+   states [n_states][n] and params [n_params][n] in, rates [n_states][n] and
+   algebraics [n_algebraics][n] (may be NULL) out, host buffers; exp_mode as in
+   FmsConfig. Synchronous. */
+int fms_eval_rhs(const double *states, const double *params, double t, int n, int exp_mode, double *rates,
+                 double *algebraics);
+/* n_steps Heun steps (the generated reaction step, GEN/03_generator_vc.cpp:1017-1130,
+   without stimulation) of n independent points on the device, FP64; states
+   [n_states][n] in and out (host buffers). Synchronous. */
+int fms_eval_heun(double *states, const double *params, double t0, double dt, int n_steps, int n, int exp_mode);
+
 int fms_create(const FmsConfig *config, FmsSolver **solver);
 int fms_destroy(FmsSolver *solver);
 int fms_set_stimulation(FmsSolver *solver, const FmsStimulation *stim);
@@ -187,7 +200,8 @@ int fms_set_parameters(FmsSolver *solver, const double *params, const double *pr
    Vm at the sensors is recorded after every split step of fms_adjoint_forward */
 int fms_set_sensors(FmsSolver *solver, const int *sensor_nodes, int n_sensors);
 
-/* Discrete adjoint (Hodgkin-Huxley model, FP64, current-pulse or no stimulation).
+/* Discrete adjoint (Hodgkin-Huxley model, FP64, current-pulse or no stimulation,
+   fibres of <= 2048 nodes; no counterpart in the reference).
    fms_adjoint_forward runs n_steps split steps from the current state
    (steps[k*4 + {0..3}] = {t, dt_split, dt0D, dt1D}), stores the state every
    checkpoint_interval steps and returns the sensor traces

@@ -104,6 +104,7 @@ EXPORTED_SYMBOLS = [
     "fms_last_error", "fms_set_tuning", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
+    "fms_eval_rhs", "fms_eval_heun",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -151,6 +152,8 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_set_sensors.argtypes = [v, _ip, i]
     lib.fms_adjoint_forward.argtypes = [v, _dp, i, i, i, _dp]
     lib.fms_adjoint_backward.argtypes = [v, _dp, _dp, _dp]
+    lib.fms_eval_rhs.argtypes = [_dp, _dp, d, i, i, _dp, _dp]
+    lib.fms_eval_heun.argtypes = [_dp, _dp, d, d, i, i, i]
     lib.fms_graph_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     lib.fms_autotune.argtypes = [v, _dp, i, i, i, ctypes.POINTER(FmsTuneResult), _dp, i]
     lib.fms_reference_set_dims.argtypes = [ctypes.POINTER(FmsReferenceDims)]
@@ -186,6 +189,34 @@ def model_info(lib: ctypes.CDLL) -> tuple[int, int, int, str]:
     buf = ctypes.create_string_buffer(256)
     check(lib, lib.fms_model_info(ctypes.byref(ns), ctypes.byref(na), ctypes.byref(nparam), buf, 256), "fms_model_info")
     return ns.value, na.value, nparam.value, buf.value.decode()
+
+
+def eval_rhs(lib: ctypes.CDLL, states: np.ndarray, params: np.ndarray, t: float = 0.0, exp_mode: str = "approx",
+             algebraics: bool = False):
+    """The model's right-hand side on the device: states (n_states, n), params (n_params, n)
+    -> rates (n_states, n) [, algebraics (n_algebraics, n)]"""
+    ns, na, nparams, _ = model_info(lib)
+    y = np.ascontiguousarray(states, np.float64).reshape(ns, -1)
+    n = y.shape[1]
+    p = np.ascontiguousarray(params, np.float64).reshape(max(nparams, 1), -1) if nparams else np.zeros((1, n))
+    r = np.empty((ns, n))
+    a = np.empty((na, n)) if algebraics else None
+    check(lib, lib.fms_eval_rhs(as_ptr(y, _dp), as_ptr(p, _dp), float(t), n,
+                                FMS_EXP_APPROXIMATE if exp_mode == "approx" else FMS_EXP_LIBM, as_ptr(r, _dp),
+                                as_ptr(a, _dp) if a is not None else None), "fms_eval_rhs")
+    return (r, a) if algebraics else r
+
+
+def eval_heun(lib: ctypes.CDLL, states: np.ndarray, params: np.ndarray, t0: float, dt: float, n_steps: int,
+              exp_mode: str = "approx") -> np.ndarray:
+    """n_steps Heun steps of independent points on the device: states (n_states, n) -> (n_states, n)"""
+    ns, _, nparams, _ = model_info(lib)
+    y = np.array(states, np.float64, copy=True).reshape(ns, -1)
+    n = y.shape[1]
+    p = np.ascontiguousarray(params, np.float64).reshape(max(nparams, 1), -1) if nparams else np.zeros((1, n))
+    check(lib, lib.fms_eval_heun(as_ptr(y, _dp), as_ptr(p, _dp), float(t0), float(dt), int(n_steps), n,
+                                 FMS_EXP_APPROXIMATE if exp_mode == "approx" else FMS_EXP_LIBM), "fms_eval_heun")
+    return y
 
 
 def model_ops(lib: ctypes.CDLL) -> dict:

@@ -90,98 +90,154 @@ FMS_HD void hh_heun_reverse(const double *y0, const double *p, double t0, double
   }
 }
 
-struct DevTape {
-  const double *tape;     // [K][9][N]: y at the start of half 1 (4), V before diffusion (1), y after it (4)
-  long long N;
-};
-
-// reverse of one reaction half for all nodes: which = 0 (first half, from tape y0)
-// or 1 (second half, from tape y^b)
+// current-pulse stimulus parameter of one Heun step, as the forward sets it
+// (the pulse is a constant of each Heun step; fms_fused.h fused_node_half)
 template <class E>
-__global__ __launch_bounds__(256) void k_adj_heun(DevFibers fb, DevStim st, DevTape tp, int s, int which, double t0,
-                                                  double dt, int n0D, double *lam, double *gacc) {
-  const long long N = fb.n_total;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int f = fb.node_fiber[i];
-  const int li = (int)(i - fb.fiber_off[f]);
-  const double *base = tp.tape + (long long)s * 9 * N + (which ? 5 * N : 0);
-  double y0[4] = {base[i], base[N + i], base[2 * N + i], base[3 * N + i]};
-  double p[4];
-  for (int k = 0; k < 4; k++) p[k] = fb.params[(long long)f * 4 + k];
-  // current pulses: the parameter during the pulse (a constant of each Heun step)
-  const bool inset = li >= fb.stim_begin[f] && li < fb.stim_end[f];
-  double l[4] = {lam[i], lam[N + i], lam[2 * N + i], lam[3 * N + i]};
-  double g[3] = {0, 0, 0};
+FMS_HD void stim_params(const DevStim &st, int f, bool inset, const double *p, double t, double *pj) {
+  for (int k = 0; k < 4; k++) pj[k] = p[k];
+  if (st.mode == 2) pj[st.stim_param] = (current_on(st, f, t) && inset) ? st.amplitude[f] : p[st.stim_param];
+}
+
+// forward reaction half of one node (n0D Heun steps from t0), the forward's arithmetic
+template <class E>
+__device__ __forceinline__ void forward_half(const DevStim &st, int f, bool inset, double *y, const double *p,
+                                             double t0, double dt, int n0D) {
+  for (int j = 0; j < n0D; j++) {
+    double pj[4];
+    stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
+    heun_node<ModelHH, double, E, false>(y[0], y + 1, pj, t0 + j * dt, dt, false, 0.0);
+  }
+}
+
+// reverse of one reaction half of one node starting from state y0 at t0:
+// l[4] in: adjoint of the half's output, out: adjoint of y0; g[3] += dL/d(gNa, gK, gL)
+template <class E>
+__device__ __forceinline__ void reverse_half(const DevStim &st, int f, bool inset, const double *y0, const double *p,
+                                             double t0, double dt, int n0D, double *l, double *g) {
   if (st.mode == 2 && inset) {
-    // pulses switch within a half step only at Heun-step boundaries: reverse step by step
+    // the parameter may switch between Heun steps: reverse step by step
     for (int j = n0D - 1; j >= 0; j--) {
       double y[4] = {y0[0], y0[1], y0[2], y0[3]};
-      double pj[4] = {p[0], p[1], p[2], p[3]};
-      for (int q = 0; q < j; q++) {
-        double pq[4] = {p[0], p[1], p[2], p[3]};
-        pq[st.stim_param] = current_on(st, f, t0 + q * dt) ? st.amplitude[f] : p[st.stim_param];
-        heun_node<ModelHH, double, E, false>(y[0], y + 1, pq, t0 + q * dt, dt, false, 0.0);
-      }
-      pj[st.stim_param] = current_on(st, f, t0 + j * dt) ? st.amplitude[f] : p[st.stim_param];
+      forward_half<E>(st, f, inset, y, p, t0, dt, j);
+      double pj[4];
+      stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
       hh_heun_reverse<E>(y, pj, t0 + j * dt, dt, 1, l, g);
     }
   } else {
     hh_heun_reverse<E>(y0, p, t0, dt, n0D, l, g);
   }
-  lam[i] = l[0]; lam[N + i] = l[1]; lam[2 * N + i] = l[2]; lam[3 * N + i] = l[3];
-  gacc[i] += g[0]; gacc[N + i] += g[1]; gacc[2 * N + i] += g[2];
 }
 
-// lambda_V[sensor] += dL/dV(sensor) of split step s
-__global__ void k_adj_sensors(DevFibers fb, const int *sensor_nodes, int n_sensors, const double *dtrace, int s,
-                              double *lam) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= fb.n_fibers * n_sensors) return;
-  const int f = idx / n_sensors;
-  lam[fb.fiber_off[f] + sensor_nodes[idx]] += dtrace[(long long)s * fb.n_fibers * n_sensors + idx];
-}
-
-// reverse diffusion of one fibre per workgroup (Vm adjoint in LDS, partitioned solve)
-template <int T, int MMAX>
-__global__ __launch_bounds__(T) void k_adj_thomas(DevFibers fb, DevTape tp, int s, double dt, int ie, double *lam,
-                                                  double *pf_bar) {
+// Reverse of kc split steps of one segment, one workgroup per fibre, steps
+// s = kc-1 .. 0 (rows of `steps`, 4 doubles each). Per step: the sensors'
+// dL/dV, reverse of half 2 from y^b = (V^b, gates after half 1; the gates are
+// recomputed from the tape's y0), reverse diffusion w = A^{-1} lambda_V,
+// lambda_V = B w (A, B symmetric), pf_bar += w^T (dB/dpf V^a - dA/dpf V^b),
+// reverse of half 1 from y0. lambda and the per-node parameter gradients stay
+// in registers for the whole segment. Tape: [kc][6][N] = y0 (4), V^a, V^b.
+template <class E, int T, int NPT, int MMAX>
+__global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const double *__restrict__ tape,
+                                                 const double *__restrict__ steps, int kc, int n0D, int ie,
+                                                 const int *__restrict__ sensor_slot,
+                                                 const double *__restrict__ dtrace, int n_sensors, double *lam,
+                                                 double *gacc, double *pf_bar) {
   extern __shared__ double sh[];
   const int f = blockIdx.x;
   const int tid = threadIdx.x;
+  const long long N = fb.n_total;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
-  const long long N = fb.n_total;
   const double *h = fb.elem_len + off - f;
   const double pf = fb.prefactor[f];
-  double *red = sh;
-  double *lw = sh + 16 * T;        // n: rhs lambda_b, then w
-  double *lr = lw + n;             // n: result B w
-  double *part = lr + n;           // T partial sums
-  for (int i = tid; i < n; i += T) lw[i] = lam[off + i];
-  __syncthreads();
-  lds_thomas_part<T, MMAX, true>(lw, n, h, pf, dt, ie != 0, red, tid);   // w = A^{-1} lambda_b
-  const double *Va = tp.tape + (long long)s * 9 * N + 4 * N + off;
-  const double *Vb = tp.tape + (long long)s * 9 * N + 5 * N + off;
-  double acc = 0;
-  for (int i = tid; i < n; i += T) {
-    double a, b, c, d;
-    assemble_row(i, n, lw, h, pf, dt, ie != 0, a, b, c, d);   // d = (B w)_i
-    lr[i] = d;
-    // (K1 v)_i for v = V^b (IE) or V^a + V^b (CN)
-    double k1v = 0;
-    const double vi = ie ? Vb[i] : Va[i] + Vb[i];
-    if (i > 0) k1v += ((ie ? Vb[i - 1] : Va[i - 1] + Vb[i - 1]) - vi) / h[i - 1];
-    if (i < n - 1) k1v += ((ie ? Vb[i + 1] : Va[i + 1] + Vb[i + 1]) - vi) / h[i];
-    acc += lw[i] * (ie ? -k1v : -0.5 * k1v);
+  double *red = sh;          // 16*T: partitioned solve
+  double *lw = sh + 16 * T;  // n: lambda_V, then w
+  double p[4];
+  for (int k = 0; k < 4; k++) p[k] = fb.params[(long long)f * 4 + k];
+  const int sb = fb.stim_begin[f], se = fb.stim_end[f];
+  double l[NPT][4], g[NPT][3];
+  double pfacc = 0;
+#pragma unroll
+  for (int q = 0; q < NPT; q++) {
+    const int i = tid + q * T;
+    if (i < n) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) l[q][k] = lam[k * N + off + i];
+#pragma unroll
+      for (int k = 0; k < 3; k++) g[q][k] = gacc[k * N + off + i];
+    }
   }
-  part[tid] = acc;
-  __syncthreads();
-  for (int i = tid; i < n; i += T) lam[off + i] = lr[i];
-  if (tid == 0) {
-    double sum = 0;
-    for (int q = 0; q < T; q++) sum += part[q];
-    pf_bar[f] += sum;
+  for (int s = kc - 1; s >= 0; s--) {
+    const double *row = steps + 4 * s;
+    const double t = row[0], dts = row[1], dt0 = row[2], dt1 = row[3];
+    const double *tp = tape + (long long)s * 6 * N;
+    // sensors, reverse of half 2
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+      const int i = tid + q * T;
+      if (i < n) {
+        const long long node = off + i;
+        const bool inset = i >= sb && i < se;
+        if (sensor_slot) {
+          const int slot = sensor_slot[node];
+          if (slot >= 0) l[q][0] += dtrace[((long long)s * fb.n_fibers + f) * n_sensors + slot];
+        }
+        double y[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
+        forward_half<E>(st, f, inset, y, p, t, dt0, n0D);
+        y[0] = tp[5 * N + node];
+        reverse_half<E>(st, f, inset, y, p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
+        lw[i] = l[q][0];
+      }
+    }
+    __syncthreads();
+    // reverse diffusion
+    lds_thomas_part<T, MMAX, true>(lw, n, h, pf, dt1, ie != 0, red, tid);
+    const double *Va = tp + 4 * N + off, *Vb = tp + 5 * N + off;
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+      const int i = tid + q * T;
+      if (i < n) {
+        double a, b, c, d;
+        assemble_row(i, n, lw, h, pf, dt1, ie != 0, a, b, c, d);   // d = (B w)_i
+        const double vi = ie ? Vb[i] : Va[i] + Vb[i];
+        double k1v = 0;
+        if (i > 0) k1v += ((ie ? Vb[i - 1] : Va[i - 1] + Vb[i - 1]) - vi) / h[i - 1];
+        if (i < n - 1) k1v += ((ie ? Vb[i + 1] : Va[i + 1] + Vb[i + 1]) - vi) / h[i];
+        pfacc += lw[i] * (ie ? -k1v : -0.5 * k1v);
+        l[q][0] = d;
+      }
+    }
+    // this step's prefactor gradient, summed in a fixed order and added to the
+    // fibre's total step by step (independent of the checkpoint segmentation)
+    red[tid] = pfacc;
+    pfacc = 0;
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0;
+      for (int q = 0; q < T; q++) sum += red[q];
+      pf_bar[f] += sum;
+    }
+    __syncthreads();
+    // reverse of half 1
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+      const int i = tid + q * T;
+      if (i < n) {
+        const long long node = off + i;
+        const bool inset = i >= sb && i < se;
+        const double y0[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
+        reverse_half<E>(st, f, inset, y0, p, t, dt0, n0D, l[q], g[q]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NPT; q++) {
+    const int i = tid + q * T;
+    if (i < n) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) lam[k * N + off + i] = l[q][k];
+#pragma unroll
+      for (int k = 0; k < 3; k++) gacc[k * N + off + i] = g[q][k];
+    }
   }
 }
 

@@ -33,14 +33,14 @@ struct DevTrace {
   const int *sensor_slot;   // [N] sensor number of each node, -1 elsewhere; nullptr: no traces
   double *trace;            // [K][F][n_sensors], this launch's first step first
   int n_sensors;
-  double *tape;             // [K][2*NS+1][N]: y before half 1 (NS), Vm before diffusion (1),
-                            // y after it (NS); nullptr: none
+  double *tape;             // [K][NS+2][N]: y before half 1 (NS), Vm before the diffusion (1),
+                            // Vm after it (1); nullptr: none
 };
 
 template <int NS, class G>
 __device__ __forceinline__ void tape_store(const DevTrace &tr, long long N, int s, int slot0, long long i, double V,
                                            const G *g, int nstore) {
-  double *t = tr.tape + ((long long)s * (2 * NS + 1) + slot0) * N + i;
+  double *t = tr.tape + ((long long)s * (NS + 2) + slot0) * N + i;
   t[0] = V;
   for (int k = 1; k < nstore; k++) t[(long long)k * N] = (double)g[k - 1];
 }
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     __syncthreads();
 
     auto node = [&](int i, double &V, G *g, double &vp, int &sf, int &sc) {
-      if (tr.tape) tape_store<M::NS>(tr, N, s, M::NS + 1, off + i, V, g, M::NS);
+      if (tr.tape) tape_store<M::NS>(tr, N, s, M::NS + 1, off + i, V, g, 1);
       fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim);
       if (tr.sensor_slot) {
         const int slot = tr.sensor_slot[off + i];

@@ -464,6 +464,43 @@ __global__ __launch_bounds__(T) void k_thomas_part(DevFibers fb, double *vm, Ste
   for (int i = tid; i < n; i += T) vm[off + i] = lv[i];
 }
 
+// model right-hand side at n independent points (SoA in and out)
+template <class M, class E>
+__global__ __launch_bounds__(256) void k_eval_rhs(const double *__restrict__ ys, const double *__restrict__ ps,
+                                                  double t, int n, double *__restrict__ rs, double *__restrict__ as) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double y[M::NS], r[M::NS], a[M::NA], p[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+  for (int k = 0; k < M::NS; k++) y[k] = ys[(long long)k * n + i];
+#pragma unroll
+  for (int k = 0; k < M::NP; k++) p[k] = ps[(long long)k * n + i];
+  M::template rhs<double, E>(y, p, t, r, a);
+#pragma unroll
+  for (int k = 0; k < M::NS; k++) rs[(long long)k * n + i] = r[k];
+  if (as)
+#pragma unroll
+    for (int k = 0; k < M::NA; k++) as[(long long)k * n + i] = a[k];
+}
+
+// n_steps Heun steps at n independent points (SoA in place)
+template <class M, class E>
+__global__ __launch_bounds__(256) void k_eval_heun(double *__restrict__ ys, const double *__restrict__ ps,
+                                                   double t0, double dt, int n_steps, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double V = ys[i];
+  double g[M::NS - 1], p[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) g[k - 1] = ys[(long long)k * n + i];
+#pragma unroll
+  for (int k = 0; k < M::NP; k++) p[k] = ps[(long long)k * n + i];
+  for (int j = 0; j < n_steps; j++) heun_node<M, double, E, false>(V, g, p, t0 + j * dt, dt, false, 0.0);
+  ys[i] = V;
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) ys[(long long)k * n + i] = g[k - 1];
+}
+
 // spike detector: upward crossings of the threshold after each split step
 __global__ void k_spikes(long long N, const double *vm, double *vm_prev, int *first_step, int *count, double th,
                          StepRef sr) {

@@ -514,6 +514,62 @@ int fms_model_defaults(double *states, double *params) {
   return 0;
 }
 
+int fms_eval_rhs(const double *states, const double *params, double t, int n, int exp_mode, double *rates,
+                 double *algebraics) {
+  if (n <= 0 || !states || !rates || (Model::NP > 0 && !params)) return fail(FMS_ERR_ARGUMENT, "bad arguments");
+  if (exp_mode != FMS_EXP_APPROXIMATE && exp_mode != FMS_EXP_LIBM) return fail(FMS_ERR_ARGUMENT, "bad exp_mode");
+  const size_t NS = Model::NS, NA = Model::NA, NP = Model::NP > 0 ? Model::NP : 1;
+  double *d_y = nullptr, *d_p = nullptr, *d_r = nullptr, *d_a = nullptr;
+  int rc = 0;
+  if (dalloc(&d_y, NS * n) || dalloc(&d_p, NP * n) || dalloc(&d_r, NS * n) || (algebraics && dalloc(&d_a, NA * n)))
+    rc = FMS_ERR_HIP;
+  if (!rc && hipMemcpy(d_y, states, NS * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = FMS_ERR_HIP;
+  if (!rc && Model::NP > 0 && hipMemcpy(d_p, params, NP * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = FMS_ERR_HIP;
+  if (!rc) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (exp_mode == FMS_EXP_APPROXIMATE)
+      hipLaunchKernelGGL((fms::k_eval_rhs<Model, fms::ExpApprox>), dim3(blocks), dim3(256), 0, 0, d_y, d_p, t, n,
+                         d_r, d_a);
+    else
+      hipLaunchKernelGGL((fms::k_eval_rhs<Model, fms::ExpLibm>), dim3(blocks), dim3(256), 0, 0, d_y, d_p, t, n,
+                         d_r, d_a);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = FMS_ERR_HIP;
+  }
+  if (!rc && hipMemcpy(rates, d_r, NS * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = FMS_ERR_HIP;
+  if (!rc && algebraics && hipMemcpy(algebraics, d_a, NA * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = FMS_ERR_HIP;
+  for (double *q : {d_y, d_p, d_r, d_a})
+    if (q) (void)hipFree(q);
+  return rc ? fail(rc, "fms_eval_rhs: device error") : 0;
+}
+
+int fms_eval_heun(double *states, const double *params, double t0, double dt, int n_steps, int n, int exp_mode) {
+  if (n <= 0 || n_steps < 0 || !states || (Model::NP > 0 && !params)) return fail(FMS_ERR_ARGUMENT, "bad arguments");
+  if (exp_mode != FMS_EXP_APPROXIMATE && exp_mode != FMS_EXP_LIBM) return fail(FMS_ERR_ARGUMENT, "bad exp_mode");
+  const size_t NS = Model::NS, NP = Model::NP > 0 ? Model::NP : 1;
+  double *d_y = nullptr, *d_p = nullptr;
+  int rc = 0;
+  if (dalloc(&d_y, NS * n) || dalloc(&d_p, NP * n)) rc = FMS_ERR_HIP;
+  if (!rc && hipMemcpy(d_y, states, NS * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = FMS_ERR_HIP;
+  if (!rc && Model::NP > 0 && hipMemcpy(d_p, params, NP * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = FMS_ERR_HIP;
+  if (!rc) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (exp_mode == FMS_EXP_APPROXIMATE)
+      hipLaunchKernelGGL((fms::k_eval_heun<Model, fms::ExpApprox>), dim3(blocks), dim3(256), 0, 0, d_y, d_p, t0, dt,
+                         n_steps, n);
+    else
+      hipLaunchKernelGGL((fms::k_eval_heun<Model, fms::ExpLibm>), dim3(blocks), dim3(256), 0, 0, d_y, d_p, t0, dt,
+                         n_steps, n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = FMS_ERR_HIP;
+  }
+  if (!rc && hipMemcpy(states, d_y, NS * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = FMS_ERR_HIP;
+  for (double *q : {d_y, d_p})
+    if (q) (void)hipFree(q);
+  return rc ? fail(rc, "fms_eval_heun: device error") : 0;
+}
+
 int fms_create(const FmsConfig *c, FmsSolver **out) {
   if (!c || !out) return fail(FMS_ERR_ARGUMENT, "null argument");
   *out = nullptr;
@@ -983,41 +1039,36 @@ template <class E> int adjoint_backward(FmsSolver *s, const double *dtraces, dou
     HIP_TRY(hipMemsetAsync(s->d_lam, 0, 4 * N * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->d_gacc, 0, 3 * N * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->d_pfbar, 0, F * sizeof(double), s->stream));
-    if ((rc = ensure(&s->d_tape, s->tape_cap, (long long)Kc * 9 * N))) return rc;
+    if ((rc = ensure(&s->d_tape, s->tape_cap, (long long)Kc * (Model::NS + 2) * N))) return rc;
     if ((rc = upload_steps(s, s->adj_steps.data(), K))) return rc;
     const int saved_spikes = s->spikes;
     s->spikes = 0;
     const int nseg = (K + Kc - 1) / Kc;
-    fms::DevTape tp{s->d_tape, N};
-    const int threads = 256;
-    const unsigned blocks = (unsigned)((N + threads - 1) / threads);
-    const size_t lds = (size_t)(16 * PART_T + 2 * s->nmax + PART_T) * sizeof(double);
+    constexpr int T = 256;
+    const size_t lds = (size_t)(16 * T + s->nmax) * sizeof(double);
     for (int seg = nseg - 1; seg >= 0; seg--) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
+      // recompute the segment from its checkpoint, recording the tape
       if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, false))) break;
       fms::DevTrace tr{};
       tr.tape = s->d_tape;
       const long long save = s->step_counter;
       if ((rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr))) break;
       s->step_counter = save;
-      for (int q = kc - 1; q >= 0; q--) {
-        const int k = k0 + q;
-        const double *st = s->adj_steps.data() + 4 * (size_t)k;
-        const double t = st[0], dts = st[1], dt0 = st[2], dt1 = st[3];
-        hipLaunchKernelGGL(fms::k_adj_sensors, dim3((F * S + 255) / 256), dim3(256), 0, s->stream, s->fibers(),
-                           s->d_sensor_nodes, S, s->d_dtrace, k, s->d_lam);
-        hipLaunchKernelGGL((fms::k_adj_heun<E>), dim3(blocks), dim3(threads), 0, s->stream, s->fibers(), s->stim(),
-                           tp, q, 1, t + 0.5 * dts, dt0, n0D, s->d_lam, s->d_gacc);
-        hipLaunchKernelGGL((fms::k_adj_thomas<PART_T, 16>), dim3(F), dim3(PART_T), lds, s->stream, s->fibers(), tp, q,
-                           dt1, ie, s->d_lam, s->d_pfbar);
-        hipLaunchKernelGGL((fms::k_adj_heun<E>), dim3(blocks), dim3(threads), 0, s->stream, s->fibers(), s->stim(),
-                           tp, q, 0, t, dt0, n0D, s->d_lam, s->d_gacc);
-        if (hipGetLastError() != hipSuccess) {
-          rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
-          break;
-        }
+      // reverse the segment
+      const double *dtr = s->d_dtrace + (size_t)k0 * F * S;
+      if (s->nmax <= 4 * T)
+        hipLaunchKernelGGL((fms::k_adj_fused<E, T, 4, 8>), dim3(F), dim3(T), lds, s->stream, s->fibers(), s->stim(),
+                           s->d_tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
+                           s->d_gacc, s->d_pfbar);
+      else
+        hipLaunchKernelGGL((fms::k_adj_fused<E, T, 8, 8>), dim3(F), dim3(T), lds, s->stream, s->fibers(), s->stim(),
+                           s->d_tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
+                           s->d_gacc, s->d_pfbar);
+      if (hipGetLastError() != hipSuccess) {
+        rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
+        break;
       }
-      if (rc) break;
     }
     s->spikes = saved_spikes;
     if (rc) return rc;
@@ -1078,6 +1129,7 @@ int fms_adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D,
   if (s->stim_mode == FMS_STIM_FIRING_TABLE)
     return fail(FMS_ERR_UNSUPPORTED, "the Vm clamp of firing-table stimulation is not differentiable; use current pulses");
   if (s->only_if_stimulated) return fail(FMS_ERR_UNSUPPORTED, "onlyComputeIfHasBeenStimulated is not differentiable");
+  if (s->nmax > 2048) return fail(FMS_ERR_UNSUPPORTED, "the adjoint takes fibres of <= 2048 nodes");
   const long long N = s->N;
   const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
   const int nseg = (K + Kc - 1) / Kc;
@@ -1214,7 +1266,9 @@ std::vector<Schedule> candidates(const FmsSolver *s, int allow_split) {
       if (small) {
         v.push_back({FMS_KERNELS_FUSED, chunk, 1, 256, 0});
         v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0});
-        if (!(s->thomas == FMS_THOMAS_SERIAL && s->nmax > 1024)) {
+        // the partitioned solve's elimination order depends on the threads per
+        // fibre, so 128-thread schedules are bit-identical only with the serial solve
+        if (s->thomas == FMS_THOMAS_SERIAL && s->nmax <= 1024) {
           v.push_back({FMS_KERNELS_FUSED, chunk, 1, 128, 0});
           v.push_back({FMS_KERNELS_FUSED, chunk, 0, 128, 0});
         }

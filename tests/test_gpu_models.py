@@ -85,3 +85,55 @@ def test_shorten_mixed_precision():
     shift = int(np.max(np.abs(a[fired] - b[fired]))) if fired.any() else 0
     print(f"shorten mixed: spike-time error {shift} steps, Vm max abs err {np.max(np.abs(g.vm() - o.vm())):.3e} mV")
     assert shift <= 1
+
+
+def _random_points(lib_name, oracle_model, n=4000, seed=0):
+    import ctypes
+    from opendihu_amd import _lib as L
+    from oracle_binding import oracle_library, P, _dp
+    g = L.load_library(lib_name)
+    o = oracle_library(oracle_model)
+    ns, na, npar, _ = L.model_info(g)
+    y0 = np.zeros(ns)
+    o.fmso_initial_states(P(y0, _dp))
+    p0 = np.zeros(max(npar, 1))
+    g.fms_model_defaults(None, P(p0, _dp))
+    rng = np.random.default_rng(seed)
+    Y = y0[:, None] * (1 + 0.02 * rng.standard_normal((ns, n)))
+    Y[0] = rng.uniform(-90, 40, n)
+    Pm = np.repeat(p0[:npar, None], n, axis=1) if npar else np.zeros((0, n))
+    return L, g, o, P, _dp, Y, Pm
+
+
+@pytest.mark.parametrize("which", ["hh", SHORTEN])
+@pytest.mark.parametrize("exp_mode", ["approx", "libm"])
+def test_device_rhs_and_heun_match_oracle(which, exp_mode):
+    """the compiled model evaluated on the device at random states: rates and ten
+    Heun steps bit-identical to the oracle with the approximate exponential; with
+    the math library exp/log (device libm vs glibc) within 1e-10 relative"""
+    lib = "libfms_hodgkin_huxley_1952.so" if which == "hh" else _lib_or_skip(f"libfms_{SHORTEN}.so")
+    if which != "hh" and not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}.so")):
+        pytest.skip("oracle Shorten model not built")
+    L, g, o, P, _dp, Y, Pm = _random_points(lib, which)
+    mode = 0 if exp_mode == "approx" else 1
+    n = Y.shape[1]
+    r = L.eval_rhs(g, Y, Pm, 0.0, exp_mode)
+    yh = L.eval_heun(g, Y, Pm, 0.0, 2e-4, 10, exp_mode)
+    ro, yo = np.empty_like(r), np.empty_like(yh)
+    rr = np.zeros(Y.shape[0])
+    aa = np.zeros(256)
+    for j in range(n):
+        yy = np.ascontiguousarray(Y[:, j])
+        pp = np.ascontiguousarray(Pm[:, j]) if Pm.shape[0] else np.zeros(1)
+        o.fmso_rhs(P(yy, _dp), P(pp, _dp), 0.0, P(rr, _dp), P(aa, _dp), mode)
+        ro[:, j] = rr
+        o.fmso_heun_cell(P(yy, _dp), P(pp, _dp), 0.0, 2e-4, 10, mode)
+        yo[:, j] = yy
+    if exp_mode == "approx":
+        np.testing.assert_array_equal(r, ro)
+        np.testing.assert_array_equal(yh, yo)
+    else:
+        np.testing.assert_allclose(r, ro, rtol=1e-10, atol=1e-300)
+        np.testing.assert_allclose(yh, yo, rtol=1e-10, atol=1e-12)
+    # repeated device evaluation gives the same bits
+    np.testing.assert_array_equal(L.eval_rhs(g, Y, Pm, 0.0, exp_mode), r)

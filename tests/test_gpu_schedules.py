@@ -62,7 +62,9 @@ def test_graph_replay_equals_eager(thomas):
     assert runs["eager"][3]["replays"] == 0
 
 
-def test_graph_recaptured_after_stimulation_change():
+def test_graph_follows_stimulation_and_spike_changes():
+    """new stimulation values are read from device memory by the same graph;
+    switching the spike detector on changes the launch sequence and recaptures"""
     mesh, stim = uneven(6, lo=300, hi=500, seed=3)
     out = []
     for gs in (0, 16):
@@ -73,14 +75,21 @@ def test_graph_recaptured_after_stimulation_change():
         st2 = dict(stim, onset=stim["onset"] + 0.5, amplitude=stim["amplitude"] * 0.8)
         g.set_current_stimulation(**st2)
         g.advance(0.064, 1e-3, 640)
-        out.append((g.get_states(), g.graph_counts()))
+        c1 = g.graph_counts()
+        g.enable_spike_detection(0.0)
+        g.advance(0.704, 1e-3, 300)
+        out.append((g.get_states(), g.spikes(), c1, g.graph_counts()))
     np.testing.assert_array_equal(out[0][0], out[1][0])
-    assert out[1][1]["captures"] == 2
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)
+    assert out[1][2]["captures"] == 1 and out[1][3]["captures"] == 2
 
 
 @pytest.mark.parametrize("thomas", ["serial", "partitioned"])
 @pytest.mark.parametrize("resident", [0, 1])
 def test_fused_128_threads_equals_256(thomas, resident):
+    """serial solve: bit-identical; partitioned solve: the partition follows the
+    thread count (another elimination order), so the bar is the partitioned one"""
     mesh, stim = uneven(10, lo=300, hi=1000 if thomas == "serial" else 2000, seed=8)
     runs = []
     for threads in (256, 128):
@@ -90,7 +99,12 @@ def test_fused_128_threads_equals_256(thomas, resident):
         g.enable_spike_detection(0.0)
         g.advance(0.0, 1e-3, 700)
         runs.append((g.get_states(), g.spikes()))
-    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    if thomas == "serial":
+        np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    else:
+        err = np.linalg.norm(runs[0][0][0] - runs[1][0][0]) / np.linalg.norm(runs[0][0][0])
+        print(f"128 vs 256 threads, partitioned: Vm rel L2 {err:.3e}")
+        assert err <= 1e-10
     np.testing.assert_array_equal(runs[0][1][0], runs[1][1][0])
 
 
