@@ -135,15 +135,16 @@ def cpu_baseline(mesh, stim, scheme, model, dt, budget_s=12.0):
                        f"({el:.1f} s, FP64 oracle, {threads} OpenMP threads on {cpu})")
 
 
-def _spike_shift(first_a, first_b):
-    """largest difference in first-spike step over nodes that fired in either run"""
-    fired = (first_a >= 0) | (first_b >= 0)
-    if not fired.any():
-        return 0, 0
-    big = 1 << 30
-    a = np.where(first_a >= 0, first_a, big)
-    b = np.where(first_b >= 0, first_b, big)
-    return int(np.max(np.abs(a[fired] - b[fired]))), int(fired.sum())
+def _spike_shift(first_a, first_b, n_steps):
+    """first-spike step differences: largest shift over nodes that fired in both
+    runs, and the nodes that fired in one run only (with the earliest such step,
+    which shows whether they are crossings at the end of the window)"""
+    both = (first_a >= 0) & (first_b >= 0)
+    one = (first_a >= 0) != (first_b >= 0)
+    shift = int(np.max(np.abs(first_a[both] - first_b[both]))) if both.any() else 0
+    only = np.where(first_a >= 0, first_a, first_b)[one]
+    return dict(spike_time_error_steps=shift, nodes_fired_both=int(both.sum()), nodes_fired_one_run_only=int(one.sum()),
+                earliest_one_run_only_step=int(only.min()) if one.any() else None, steps=n_steps)
 
 
 def bench_c3(cpu=True):
@@ -173,14 +174,14 @@ def bench_c3(cpu=True):
         res[precision] = dict(value=g.n_total * n_steps / wall, ms_per_step=wall / n_steps * 1e3,
                               event_ms=g.elapsed_ms(0, 1), schedule=choice, vm=g.vm(), first=g.spikes()[0])
         g.close()
-    shift, fired = _spike_shift(res["mixed"]["first"], res["fp64"]["first"])
+    spikes = _spike_shift(res["mixed"]["first"], res["fp64"]["first"], n_steps)
     vm_err = float(np.max(np.abs(res["mixed"]["vm"] - res["fp64"]["vm"])))
     out = dict(workload="hh_4096_uneven_400-2000_CN_dt1e-3_10ms", nodes=int(mesh["n_nodes"].sum()), steps=n_steps,
                value_fp64=res["fp64"]["value"], value_mixed=res["mixed"]["value"],
                ms_per_step_fp64=res["fp64"]["ms_per_step"], ms_per_step_mixed=res["mixed"]["ms_per_step"],
                schedule_fp64={k: v for k, v in res["fp64"]["schedule"].items()},
                schedule_mixed={k: v for k, v in res["mixed"]["schedule"].items()},
-               mixed_vs_fp64=dict(spike_time_error_steps=shift, nodes_fired=fired, vm_max_abs_err_mV=vm_err))
+               mixed_vs_fp64=dict(spikes, vm_max_abs_err_mV=vm_err))
     if cpu:
         c = cpu_baseline(mesh, stim, "CN", "hh", dt, budget_s=10.0)
         out["cpu_baseline"] = c
@@ -229,12 +230,15 @@ def bench_c4():
     v = np.random.default_rng(43).standard_normal(theta0.shape) * theta0
     h = 1e-6
     fd = (loss(theta0 + h * v) - loss(theta0 - h * v)) / (2 * h)
+    fd2 = (loss(theta0 + 2 * h * v) - loss(theta0 - 2 * h * v)) / (4 * h)
+    fdr = (4 * fd - fd2) / 3
     ad = float(np.sum(grad * v))
     b.close()
     return dict(workload="hh_adjoint_256x1000_CN_dt1e-3_5ms_32sensors_ckpt100", steps=n_steps,
                 plain_forward_s=t_plain, recording_forward_s=t_fwd, backward_s=t_bwd,
                 gradient_cost_vs_forward=(t_fwd + t_bwd) / t_plain, backward_vs_forward=t_bwd / t_plain,
-                directional_derivative=dict(adjoint=ad, central_fd=fd, h=h, rel_err=abs(ad - fd) / abs(fd)),
+                directional_derivative=dict(adjoint=ad, central_fd=fd, h=h, rel_err=abs(ad - fd) / abs(fd),
+                                            central_fd_2h=fd2, richardson=fdr, rel_err_richardson=abs(ad - fdr) / abs(fdr)),
                 note="wall times include the host copies of the traces (5000 x 256 x 32 doubles each way)")
 
 
@@ -263,12 +267,11 @@ def bench_c5(steps=200, warmup=20):
         res[precision] = dict(value=g.n_total * steps / wall, ms_per_step=wall / steps * 1e3, vm=g.vm(),
                               first=g.spikes()[0])
         g.close()
-    shift, fired = _spike_shift(res["mixed"]["first"], res["fp64"]["first"])
+    spikes = _spike_shift(res["mixed"]["first"], res["fp64"]["first"], steps + warmup)
     return dict(workload="shorten56_16384_uneven_1000-1481_IE_dt2.5e-5_100MU", nodes=int(mesh["n_nodes"].sum()),
                 steps=steps, warmup=warmup, value_mixed=res["mixed"]["value"], value_fp64=res["fp64"]["value"],
                 ms_per_step_mixed=res["mixed"]["ms_per_step"], ms_per_step_fp64=res["fp64"]["ms_per_step"],
-                mixed_vs_fp64=dict(vm_max_abs_err_mV=float(np.max(np.abs(res["mixed"]["vm"] - res["fp64"]["vm"]))),
-                                   spike_time_error_steps=shift, nodes_fired=fired,
+                mixed_vs_fp64=dict(spikes, vm_max_abs_err_mV=float(np.max(np.abs(res["mixed"]["vm"] - res["fp64"]["vm"]))),
                                    after_steps=steps + warmup))
 
 

@@ -110,21 +110,18 @@ __device__ __forceinline__ void forward_half(const DevStim &st, int f, bool inse
 }
 
 // reverse of one reaction half of one node starting from state y0 at t0:
-// l[4] in: adjoint of the half's output, out: adjoint of y0; g[3] += dL/d(gNa, gK, gL)
+// l[4] in: adjoint of the half's output, out: adjoint of y0; g[3] += dL/d(gNa, gK, gL).
+// Heun step j is reversed from its recomputed input with the stimulus
+// parameter of that step (a per-lane select), so all lanes take the same path.
 template <class E>
 __device__ __forceinline__ void reverse_half(const DevStim &st, int f, bool inset, const double *y0, const double *p,
                                              double t0, double dt, int n0D, double *l, double *g) {
-  if (st.mode == 2 && inset) {
-    // the parameter may switch between Heun steps: reverse step by step
-    for (int j = n0D - 1; j >= 0; j--) {
-      double y[4] = {y0[0], y0[1], y0[2], y0[3]};
-      forward_half<E>(st, f, inset, y, p, t0, dt, j);
-      double pj[4];
-      stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
-      hh_heun_reverse<E>(y, pj, t0 + j * dt, dt, 1, l, g);
-    }
-  } else {
-    hh_heun_reverse<E>(y0, p, t0, dt, n0D, l, g);
+  for (int j = n0D - 1; j >= 0; j--) {
+    double y[4] = {y0[0], y0[1], y0[2], y0[3]};
+    forward_half<E>(st, f, inset, y, p, t0, dt, j);
+    double pj[4];
+    stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
+    hh_heun_reverse<E>(y, pj, t0 + j * dt, dt, 1, l, g);
   }
 }
 
@@ -156,15 +153,14 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   double l[NPT][4], g[NPT][3];
   double pfacc = 0;
+  // lanes past the fibre's end shadow its last node: they compute, never store
 #pragma unroll
   for (int q = 0; q < NPT; q++) {
-    const int i = tid + q * T;
-    if (i < n) {
+    const int i = tid + q * T < n ? tid + q * T : n - 1;
 #pragma unroll
-      for (int k = 0; k < 4; k++) l[q][k] = lam[k * N + off + i];
+    for (int k = 0; k < 4; k++) l[q][k] = lam[k * N + off + i];
 #pragma unroll
-      for (int k = 0; k < 3; k++) g[q][k] = gacc[k * N + off + i];
-    }
+    for (int k = 0; k < 3; k++) g[q][k] = gacc[k * N + off + i];
   }
   for (int s = kc - 1; s >= 0; s--) {
     const double *row = steps + 4 * s;
@@ -173,20 +169,18 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
     // sensors, reverse of half 2
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
-      const int i = tid + q * T;
-      if (i < n) {
-        const long long node = off + i;
-        const bool inset = i >= sb && i < se;
-        if (sensor_slot) {
-          const int slot = sensor_slot[node];
-          if (slot >= 0) l[q][0] += dtrace[((long long)s * fb.n_fibers + f) * n_sensors + slot];
-        }
-        double y[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
-        forward_half<E>(st, f, inset, y, p, t, dt0, n0D);
-        y[0] = tp[5 * N + node];
-        reverse_half<E>(st, f, inset, y, p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
-        lw[i] = l[q][0];
-      }
+      const bool valid = tid + q * T < n;
+      const int i = valid ? tid + q * T : n - 1;
+      const long long node = off + i;
+      const bool inset = i >= sb && i < se;
+      const int slot = sensor_slot[node];
+      const double dl = dtrace[((long long)s * fb.n_fibers + f) * n_sensors + (slot >= 0 ? slot : 0)];
+      l[q][0] += slot >= 0 ? dl : 0.0;
+      double y[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
+      forward_half<E>(st, f, inset, y, p, t, dt0, n0D);
+      y[0] = tp[5 * N + node];
+      reverse_half<E>(st, f, inset, y, p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
+      if (valid) lw[i] = l[q][0];
     }
     __syncthreads();
     // reverse diffusion
@@ -194,17 +188,16 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
     const double *Va = tp + 4 * N + off, *Vb = tp + 5 * N + off;
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
-      const int i = tid + q * T;
-      if (i < n) {
-        double a, b, c, d;
-        assemble_row(i, n, lw, h, pf, dt1, ie != 0, a, b, c, d);   // d = (B w)_i
-        const double vi = ie ? Vb[i] : Va[i] + Vb[i];
-        double k1v = 0;
-        if (i > 0) k1v += ((ie ? Vb[i - 1] : Va[i - 1] + Vb[i - 1]) - vi) / h[i - 1];
-        if (i < n - 1) k1v += ((ie ? Vb[i + 1] : Va[i + 1] + Vb[i + 1]) - vi) / h[i];
-        pfacc += lw[i] * (ie ? -k1v : -0.5 * k1v);
-        l[q][0] = d;
-      }
+      const bool valid = tid + q * T < n;
+      const int i = valid ? tid + q * T : n - 1;
+      double a, b, c, d;
+      assemble_row(i, n, lw, h, pf, dt1, ie != 0, a, b, c, d);   // d = (B w)_i
+      const double vi = ie ? Vb[i] : Va[i] + Vb[i];
+      double k1v = 0;
+      if (i > 0) k1v += ((ie ? Vb[i - 1] : Va[i - 1] + Vb[i - 1]) - vi) / h[i - 1];
+      if (i < n - 1) k1v += ((ie ? Vb[i + 1] : Va[i + 1] + Vb[i + 1]) - vi) / h[i];
+      pfacc += valid ? lw[i] * (ie ? -k1v : -0.5 * k1v) : 0.0;
+      l[q][0] = d;
     }
     // this step's prefactor gradient, summed in a fixed order and added to the
     // fibre's total step by step (independent of the checkpoint segmentation)
@@ -220,13 +213,11 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
     // reverse of half 1
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
-      const int i = tid + q * T;
-      if (i < n) {
-        const long long node = off + i;
-        const bool inset = i >= sb && i < se;
-        const double y0[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
-        reverse_half<E>(st, f, inset, y0, p, t, dt0, n0D, l[q], g[q]);
-      }
+      const int i = tid + q * T < n ? tid + q * T : n - 1;
+      const long long node = off + i;
+      const bool inset = i >= sb && i < se;
+      const double y0[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
+      reverse_half<E>(st, f, inset, y0, p, t, dt0, n0D, l[q], g[q]);
     }
   }
 #pragma unroll

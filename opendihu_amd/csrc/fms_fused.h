@@ -45,18 +45,65 @@ __device__ __forceinline__ void tape_store(const DevTrace &tr, long long N, int 
   for (int k = 1; k < nstore; k++) t[(long long)k * N] = (double)g[k - 1];
 }
 
+// tape entry with the gates read from their HBM copy (streamed path); g == nullptr
+// stores Vm only
+template <int NS, class G>
+__device__ __forceinline__ void tape_store_streamed(const DevTrace &tr, long long N, int s, int slot0, long long i,
+                                                    double V, const G *gates) {
+  double *t = tr.tape + ((long long)s * (NS + 2) + slot0) * N + i;
+  t[0] = V;
+  if (gates)
+    for (int k = 1; k < NS; k++) t[(long long)k * N] = (double)gates[(long long)(k - 1) * N + i];
+}
+
 template <class M, class G, class E>
 __device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, double pbase, const DevStim &st,
                                                 bool inset, const unsigned char *flags, double t0, double dt,
                                                 int n0D, double vstim) {
   for (int j = 0; j < n0D; j++) {
-    if (!(flags[j] & FLAG_COMPUTE)) continue;
-    const int on = flags[j] & FLAG_STIM;
+    // one fibre per workgroup: the flags are the same on every lane
+    const int fl = __builtin_amdgcn_readfirstlane((int)flags[j]);
+    if (!(fl & FLAG_COMPUTE)) continue;
+    const int on = fl & FLAG_STIM;
     const bool clamp = st.mode == 1 && on && inset;
     // p is shared by all nodes a thread owns: set the stimulus parameter for every node
     if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[0] : pbase;
     heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
   }
+}
+
+// One reaction half of one node with its gates streamed from HBM: load the
+// gates, n0D Heun steps, store the gates (if `store`), return Vm. Not inlined:
+// for large models the half is thousands of instructions under full register
+// pressure, and compiling it once as its own function keeps the fused kernel's
+// loop, LDS and barrier code out of its register allocation (inlined into the
+// fused kernel, the 56-state model's reaction gave wrong, run-to-run different
+// results on gfx950 while the same code in the split kernel was exact).
+template <class M, class G, class E>
+__device__ __noinline__ double streamed_half(G *__restrict__ gates, long long N, long long node, bool store, double V,
+                                             const double *__restrict__ fparams, int stim_mode, int stim_param,
+                                             double amplitude, bool inset, const unsigned char *flags, double t0,
+                                             double dt, int n0D, double vstim) {
+  G g[M::NS - 1];
+#pragma unroll
+  for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + node];
+  double p[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+  for (int k = 0; k < M::NP; k++) p[k] = fparams[k];
+  const double pbase = M::NP > 0 ? p[stim_param] : 0.0;
+  for (int j = 0; j < n0D; j++) {
+    const int fl = __builtin_amdgcn_readfirstlane((int)flags[j]);
+    if (!(fl & FLAG_COMPUTE)) continue;
+    const int on = fl & FLAG_STIM;
+    const bool clamp = stim_mode == 1 && on && inset;
+    if (stim_mode == 2 && M::NP > 0) p[stim_param] = (on && inset) ? amplitude : pbase;
+    heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
+  }
+  if (store) {
+#pragma unroll
+    for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + node] = g[k];
+  }
+  return V;
 }
 
 // NPT > 0: gates resident in registers, NPT nodes per thread; NPT == 0: streamed
@@ -87,6 +134,8 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   // the current-pulse amplitude is read through a per-fibre view
   DevStim stf = st;
   if (st.mode == 2) stf.amplitude = st.amplitude + f;
+  const double *fparams = fb.params + (long long)f * (M::NP > 0 ? M::NP : 1);
+  const double amp = st.mode == 2 ? st.amplitude[f] : 0.0;
 
   G greg[NPT > 0 ? NPT : 1][M::NS - 1];
   double vprev[NPT > 0 ? NPT : 1];
@@ -94,15 +143,17 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   if (NPT > 0) {
 #pragma unroll
     for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
-      const int i = tid + q * T;
-      if (i < n) {
+      // lanes past the fibre's end shadow its last node (computed, never stored)
+      const int i = tid + q * T < n ? tid + q * T : n - 1;
 #pragma unroll
-        for (int k = 0; k < M::NS - 1; k++) greg[q][k] = gates[(long long)k * N + off + i];
-        if (spk.enabled) {
-          vprev[q] = spk.vm_prev[off + i];
-          sfirst[q] = spk.first_step[off + i];
-          scount[q] = spk.count[off + i];
-        }
+      for (int k = 0; k < M::NS - 1; k++) greg[q][k] = gates[(long long)k * N + off + i];
+      vprev[q] = 0;
+      sfirst[q] = 0;
+      scount[q] = 0;
+      if (spk.enabled) {
+        vprev[q] = spk.vm_prev[off + i];
+        sfirst[q] = spk.first_step[off + i];
+        scount[q] = spk.count[off + i];
       }
     }
   }
@@ -115,30 +166,30 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   // pass 0: first reaction half of step 0
   {
     const double t0 = steps[0], dt0 = steps[2];
+    // every lane runs the reaction: lanes past the fibre's end shadow its last
+    // node and store nothing (a shadow may read that node's Vm before or after
+    // its owner updates it; the shadow's result is discarded)
     if (NPT > 0) {
 #pragma unroll
       for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
-        const int i = tid + q * T;
-        if (i < n) {
-          double V = lv[i];
-          if (tr.tape) tape_store<M::NS>(tr, N, 0, 0, off + i, V, greg[q], M::NS);
-          fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
-          if (tr.tape) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, greg[q], 1);
-          lv[i] = V;
-        }
+        const bool valid = tid + q * T < n;
+        const int i = valid ? tid + q * T : n - 1;
+        double V = lv[i];
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, 0, off + i, V, greg[q], M::NS);
+        fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, greg[q], 1);
+        if (valid) lv[i] = V;
       }
     } else {
-      for (int i = tid; i < n; i += T) {
-        G g[M::NS - 1];
-#pragma unroll
-        for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + off + i];
+      for (int base = 0; base < n; base += T) {
+        const bool valid = base + tid < n;
+        const int i = valid ? base + tid : n - 1;
         double V = lv[i];
-        if (tr.tape) tape_store<M::NS>(tr, N, 0, 0, off + i, V, g, M::NS);
-        fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
-        if (tr.tape) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, g, 1);
-        lv[i] = V;
-#pragma unroll
-        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = g[k];
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, 0, off + i, V, gates);
+        V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
+                                   sflags, t0, dt0, n0D, vstim);
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, M::NS, off + i, V, (const G *)nullptr);
+        if (valid) lv[i] = V;
       }
     }
   }
@@ -168,60 +219,73 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     }
     __syncthreads();
 
-    auto node = [&](int i, double &V, G *g, double &vp, int &sf, int &sc) {
-      if (tr.tape) tape_store<M::NS>(tr, N, s, M::NS + 1, off + i, V, g, 1);
+    // one node: second half of step s, traces and spikes, first half of step s+1
+    // (valid = false: a shadow lane, which computes and stores nothing)
+    auto node = [&](int i, bool valid, double &V, G *g, double &vp, int &sf, int &sc) {
+      if (tr.tape && valid) tape_store<M::NS>(tr, N, s, M::NS + 1, off + i, V, g, 1);
       fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim);
-      if (tr.sensor_slot) {
+      if (tr.sensor_slot && valid) {
         const int slot = tr.sensor_slot[off + i];
         if (slot >= 0) tr.trace[((long long)s * fb.n_fibers + f) * tr.n_sensors + slot] = V;
       }
       if (spk.enabled) {
-        if (vp < spk.threshold && V >= spk.threshold) {
-          sc++;
-          if (sf < 0) sf = spk.step0 + s + 1;
-        }
+        const bool cross = vp < spk.threshold && V >= spk.threshold;
+        sc += cross ? 1 : 0;
+        sf = (cross && sf < 0) ? spk.step0 + s + 1 : sf;
         vp = V;
       }
       if (more) {
-        if (tr.tape) tape_store<M::NS>(tr, N, s + 1, 0, off + i, V, g, M::NS);
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, 0, off + i, V, g, M::NS);
         fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D,
                                  vstim);
-        if (tr.tape) tape_store<M::NS>(tr, N, s + 1, M::NS, off + i, V, g, 1);
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, M::NS, off + i, V, g, 1);
       }
     };
 
     if (NPT > 0) {
 #pragma unroll
       for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
-        const int i = tid + q * T;
-        if (i < n) {
-          double V = lv[i];
-          node(i, V, greg[q], vprev[q], sfirst[q], scount[q]);
-          lv[i] = V;
-        }
+        const bool valid = tid + q * T < n;
+        const int i = valid ? tid + q * T : n - 1;
+        double V = lv[i];
+        node(i, valid, V, greg[q], vprev[q], sfirst[q], scount[q]);
+        if (valid) lv[i] = V;
       }
     } else {
-      for (int i = tid; i < n; i += T) {
-        G g[M::NS - 1];
-#pragma unroll
-        for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + off + i];
+      // streamed gates (large models): one reaction half per pass over the
+      // nodes, so that one pass holds the registers of one half only
+      for (int base = 0; base < n; base += T) {
+        const bool valid = base + tid < n;
+        const int i = valid ? base + tid : n - 1;
         double V = lv[i];
-        double vp = 0;
-        int sf = 0, sc = 0;
-        if (spk.enabled) {
-          vp = spk.vm_prev[off + i];
-          sf = spk.first_step[off + i];
-          sc = spk.count[off + i];
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s, M::NS + 1, off + i, V, (const G *)nullptr);
+        V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
+                                   sflags, t_mid, dt0, n0D, vstim);
+        if (tr.sensor_slot && valid) {
+          const int slot = tr.sensor_slot[off + i];
+          if (slot >= 0) tr.trace[((long long)s * fb.n_fibers + f) * tr.n_sensors + slot] = V;
         }
-        node(i, V, g, vp, sf, sc);
-        lv[i] = V;
-        if (spk.enabled) {
-          spk.vm_prev[off + i] = vp;
-          spk.first_step[off + i] = sf;
-          spk.count[off + i] = sc;
+        if (spk.enabled && valid) {
+          const double vp = spk.vm_prev[off + i];
+          if (vp < spk.threshold && V >= spk.threshold) {
+            spk.count[off + i]++;
+            if (spk.first_step[off + i] < 0) spk.first_step[off + i] = spk.step0 + s + 1;
+          }
+          spk.vm_prev[off + i] = V;
         }
-#pragma unroll
-        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = g[k];
+        if (valid) lv[i] = V;
+      }
+      if (more) {
+        for (int base = 0; base < n; base += T) {
+          const bool valid = base + tid < n;
+          const int i = valid ? base + tid : n - 1;
+          double V = lv[i];
+          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, 0, off + i, V, gates);
+          V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp,
+                                     i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D, vstim);
+          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, M::NS, off + i, V, (const G *)nullptr);
+          if (valid) lv[i] = V;
+        }
       }
     }
     __syncthreads();

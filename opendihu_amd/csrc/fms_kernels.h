@@ -205,16 +205,27 @@ __global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, Ste
 
 // ----------------------------------------------------------------------------
 // reaction: one node per lane, n0D Heun steps, SoA coalesced streaming
+//
+// The heavy reaction code runs with every lane of the wave active: lanes past
+// the last node recompute the last node and do not store, and a Heun step that
+// a fibre skips (onlyComputeIfHasBeenStimulated) runs with dt = 0 on its lanes,
+// which leaves every state unchanged (y + 0*r = y), instead of branching around
+// it per lane. Waves in which no lane computes skip the step with a wave-uniform
+// branch. Per-lane branches around the large models' reaction code (thousands
+// of VALU instructions under register pressure) gave wrong and run-to-run
+// different results on gfx950; this form gives the oracle's bits.
+// ALG: also return the algebraics of the last Heun step (algebraicsForTransfer).
 // ----------------------------------------------------------------------------
-template <class M, class G, class E>
+template <class M, class G, class E, bool ALG>
 __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, double *__restrict__ vm,
                                                   G *__restrict__ gates, const unsigned char *__restrict__ flags,
                                                   StepRef sr, int half, int n0D, double vstim,
                                                   double *__restrict__ alg_transfer, const int *__restrict__ alg_idx,
                                                   int n_alg_transfer) {
   const long long N = fb.n_total;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i0 < N;
+  const long long i = valid ? i0 : N - 1;
   const int f = fb.node_fiber[i];
   const int li = (int)(i - fb.fiber_off[f]);
   const double *row = sr.row();
@@ -228,15 +239,26 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
 #pragma unroll
   for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
   const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
+  double alg[ALG ? M::NA : 1] = {};
   for (int j = 0; j < n0D; j++) {
     const unsigned char fl = flags[(long long)f * n0D + j];
-    if (!(fl & FLAG_COMPUTE)) continue;
+    const bool compute = (fl & FLAG_COMPUTE) != 0;
+    if (__builtin_amdgcn_ballot_w64(compute) == 0) continue;   // wave-uniform
     const int on = fl & FLAG_STIM;
-    const bool clamp = st.mode == 1 && on && inset;
+    const bool clamp = compute && st.mode == 1 && on && inset;
+    const double dtj = compute ? dt : 0.0;
     if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[f] : pbase;
-    if (alg_transfer != nullptr && j == n0D - 1) {
-      double alg[M::NA];
-      heun_node<M, G, E, true>(V, g, p, t0 + j * dt, dt, clamp, vstim, alg);
+    if constexpr (ALG) {
+      heun_node<M, G, E, true>(V, g, p, t0 + j * dt, dtj, clamp, vstim, alg);
+    } else {
+      heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dtj, clamp, vstim);
+    }
+  }
+  if (valid) {
+    vm[i] = V;
+#pragma unroll
+    for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + i] = g[k];
+    if constexpr (ALG) {
       for (int k = 0; k < n_alg_transfer; k++) {
         const int a = alg_idx[k];
         double v = 0;
@@ -245,13 +267,8 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
           if (q == a) v = alg[q];
         alg_transfer[(long long)k * N + i] = v;
       }
-    } else {
-      heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
     }
   }
-  vm[i] = V;
-#pragma unroll
-  for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + i] = g[k];
 }
 
 // ----------------------------------------------------------------------------

@@ -236,9 +236,14 @@ template <class G, class E>
 int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
-  hipLaunchKernelGGL((fms::k_reaction<Model, G, E>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
-                     s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                     s->vstim, store_alg ? s->d_alg : nullptr, s->d_alg_idx, s->n_alg_idx);
+  if (store_alg)
+    hipLaunchKernelGGL((fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
+                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+                       s->vstim, s->d_alg, s->d_alg_idx, s->n_alg_idx);
+  else
+    hipLaunchKernelGGL((fms::k_reaction<Model, G, E, false>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
+                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+                       s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx);
   HIP_TRY(hipGetLastError());
   s->n_launch_reaction++;
   return 0;
