@@ -209,7 +209,12 @@ def bench_c4():
     sigma = mesh["prefactor"] * AM * CM
     theta_true = np.stack([rng.uniform(100, 140, F), rng.uniform(30, 42, F), rng.uniform(0.2, 0.4, F), sigma], 1)
     theta0 = theta_true * rng.uniform(0.95, 1.05, theta_true.shape)
-    target = prob.forward(theta_true)
+    import torch
+    dev = torch.device("cuda")
+    target = prob.forward_tensor(torch.tensor(theta_true, device=dev))
+    th0 = torch.tensor(theta0, device=dev)
+    # warm-up of every path (code objects, buffers)
+    prob.backward_tensor(prob.forward_tensor(th0) - target)
     # plain forward solve (no traces, no checkpoints) at theta0
     prob.set_theta(theta0)
     for k in range(b.n_states):
@@ -219,14 +224,15 @@ def bench_c4():
     b.advance_schedule(steps)
     b.synchronize()
     t_plain = time.perf_counter() - w0
+    # gradient: recording forward + backward, traces and adjoints in device memory
     w0 = time.perf_counter()
-    tr = prob.forward(theta0)
+    tr = prob.forward_tensor(th0)
     t_fwd = time.perf_counter() - w0
     resid = tr - target
     w0 = time.perf_counter()
-    grad = prob.backward(resid)
+    grad = prob.backward_tensor(resid).cpu().numpy()
     t_bwd = time.perf_counter() - w0
-    loss = lambda th: 0.5 * float(np.sum((prob.forward(th) - target) ** 2))  # noqa: E731
+    loss = lambda th: 0.5 * float(((prob.forward_tensor(torch.tensor(th, device=dev)) - target) ** 2).sum())  # noqa: E731
     v = np.random.default_rng(43).standard_normal(theta0.shape) * theta0
     h = 1e-6
     fd = (loss(theta0 + h * v) - loss(theta0 - h * v)) / (2 * h)
@@ -239,7 +245,7 @@ def bench_c4():
                 gradient_cost_vs_forward=(t_fwd + t_bwd) / t_plain, backward_vs_forward=t_bwd / t_plain,
                 directional_derivative=dict(adjoint=ad, central_fd=fd, h=h, rel_err=abs(ad - fd) / abs(fd),
                                             central_fd_2h=fd2, richardson=fdr, rel_err_richardson=abs(ad - fdr) / abs(fdr)),
-                note="wall times include the host copies of the traces (5000 x 256 x 32 doubles each way)")
+                note="wall-clock per call; traces and their adjoints stay in device memory (torch tensors)")
 
 
 def bench_c5(steps=200, warmup=20):

@@ -213,6 +213,12 @@ int fms_set_sensors(FmsSolver *solver, const int *sensor_nodes, int n_sensors);
 int fms_adjoint_forward(FmsSolver *solver, const double *steps, int n_steps, int n0D, int checkpoint_interval,
                         double *traces);
 int fms_adjoint_backward(FmsSolver *solver, const double *dtraces, double *grad_params, double *grad_prefactor);
+/* the same with the traces, their adjoints and the gradients in device memory
+   (e.g. torch tensors on the GPU): no host copies */
+int fms_adjoint_forward_device(FmsSolver *solver, const double *steps, int n_steps, int n0D,
+                               int checkpoint_interval, double *d_traces);
+int fms_adjoint_backward_device(FmsSolver *solver, const double *d_dtraces, double *d_grad_params,
+                                double *d_grad_prefactor);
 
 /* ---- reference plugin boundary ---- */
 typedef struct {

@@ -104,10 +104,22 @@ EXPORTED_SYMBOLS = [
     "fms_last_error", "fms_set_tuning", "fms_record_event", "fms_event_elapsed_ms", "fms_launch_counts",
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
-    "fms_eval_rhs", "fms_eval_heun",
+    "fms_eval_rhs", "fms_eval_heun", "fms_adjoint_forward_device", "fms_adjoint_backward_device",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
+
+
+def _share_hip_runtime_with_torch() -> None:
+    """PyTorch-ROCm ships its own libamdhip64.so.7. Loading torch first makes the
+    library's libamdhip64.so.7 dependency resolve to that same runtime (same
+    soname), so torch tensors and the library share one HIP runtime and device
+    context in the process; loaded the other way round, the second runtime finds
+    no GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def library_path(name: str = HH_LIBRARY) -> str:
@@ -122,6 +134,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(f"fibre library {path} is not built; run `python -m opendihu_amd.build` "
                            "(or __graft_entry__.build()) first")
+    _share_hip_runtime_with_torch()
     lib = ctypes.CDLL(path)
     d, i, v = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
     lib.fms_model_info.argtypes = [_ip, _ip, _ip, ctypes.c_char_p, i]
@@ -152,6 +165,8 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_set_sensors.argtypes = [v, _ip, i]
     lib.fms_adjoint_forward.argtypes = [v, _dp, i, i, i, _dp]
     lib.fms_adjoint_backward.argtypes = [v, _dp, _dp, _dp]
+    lib.fms_adjoint_forward_device.argtypes = [v, _dp, i, i, i, v]
+    lib.fms_adjoint_backward_device.argtypes = [v, v, v, v]
     lib.fms_eval_rhs.argtypes = [_dp, _dp, d, i, i, _dp, _dp]
     lib.fms_eval_heun.argtypes = [_dp, _dp, d, d, i, i, i]
     lib.fms_graph_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]

@@ -8,7 +8,8 @@ conductivity, through the hand-written discrete adjoint of the split step
 ``problem`` (:class:`AdjointProblem`) holds the fibre batch (Hodgkin-Huxley,
 current-pulse stimulation), the initial states, the split-step table, the
 sensor nodes and the checkpoint interval. Forward and backward run on the GPU;
-the tensors crossing the autograd boundary are CPU float64 tensors.
+with theta on the GPU the traces and their adjoints stay in device memory, with
+theta on the CPU they are CPU float64 tensors.
 """
 from __future__ import annotations
 
@@ -57,6 +58,40 @@ class AdjointProblem:
                                              self.interval, L.as_ptr(out, L._dp)), "fms_adjoint_forward")
         return out
 
+    def forward_tensor(self, theta):
+        """forward with the traces in a torch tensor on theta's device (no host
+        copy of the traces when theta is a GPU tensor)"""
+        torch = _torch()
+        if not theta.is_cuda:
+            return torch.from_numpy(self.forward(theta.detach().numpy()))
+        self.set_theta(theta.detach().cpu().numpy())
+        for k in range(self.batch.n_states):
+            self.batch.set_state(k, self.initial[k])
+        out = torch.empty((self.n_steps, self.batch.n_fibers, self.n_sensors), dtype=torch.float64,
+                          device=theta.device)
+        torch.cuda.synchronize(theta.device)
+        lib = self.batch.lib
+        L.check(lib, lib.fms_adjoint_forward_device(self.batch.handle, L.as_ptr(self.steps, L._dp), self.n_steps,
+                                                    self.n0D, self.interval, ctypes.c_void_p(out.data_ptr())),
+                "fms_adjoint_forward_device")
+        return out
+
+    def backward_tensor(self, dtraces):
+        """dL/dtheta for the last forward as a torch tensor on dtraces' device"""
+        torch = _torch()
+        if not dtraces.is_cuda:
+            return torch.from_numpy(self.backward(dtraces.detach().contiguous().numpy()))
+        d = dtraces.detach().to(torch.float64).contiguous()
+        F = self.batch.n_fibers
+        gp = torch.empty((F, 3), dtype=torch.float64, device=d.device)
+        gpf = torch.empty(F, dtype=torch.float64, device=d.device)
+        torch.cuda.synchronize(d.device)
+        lib = self.batch.lib
+        L.check(lib, lib.fms_adjoint_backward_device(self.batch.handle, ctypes.c_void_p(d.data_ptr()),
+                                                     ctypes.c_void_p(gp.data_ptr()), ctypes.c_void_p(gpf.data_ptr())),
+                "fms_adjoint_backward_device")
+        return torch.cat([gp, (gpf / self.am_cm)[:, None]], dim=1)
+
     def backward(self, dtraces) -> np.ndarray:
         """dL/dtheta for the last forward; theta columns (g_Na, g_K, g_L, sigma)"""
         d = np.ascontiguousarray(dtraces, np.float64)
@@ -88,13 +123,11 @@ class MonodomainTraces:
                 @staticmethod
                 def forward(ctx, th, prob):
                     ctx.prob = prob
-                    out = prob.forward(th.detach().cpu().numpy())
-                    return torch.from_numpy(out)
+                    return prob.forward_tensor(th)
 
                 @staticmethod
                 def backward(ctx, g):
-                    grad = ctx.prob.backward(g.detach().cpu().contiguous().numpy())
-                    return torch.from_numpy(grad), None
+                    return ctx.prob.backward_tensor(g), None
 
             cls._fn = _Fn
         return cls._fn.apply(theta, problem)

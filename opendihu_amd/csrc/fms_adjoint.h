@@ -30,64 +30,100 @@ template <> struct ExpDeriv<ExpLibm> {
   static FMS_HD double d(double x, double ex) { (void)x; return ex; }
 };
 
-// J(y)^T w (out[0..3]) and f_theta(y)^T w for theta = (g_Na, g_K, g_L) (gp[0..2])
+// Linearisation of the HH right-hand side at y: the rates r(y), computed with
+// the operations of ModelHH::rhs in its order (so y* = y + dt r(y) has the
+// forward's bits), and the partial derivatives that J(y)^T w and
+// f_theta(y)^T w need, from the same exponentials.
+struct HHLin {
+  double r[4];
+  double f0V, f0m, f0h, f0n;       // d f0 / d(V, m, h, n)
+  double f1V, f1m, f2V, f2h, f3V, f3n;
+  double gna, gk, gl;              // d f0 / d(g_Na, g_K, g_L)
+  FMS_HD void jt(const double *w, double *out) const {
+    out[0] = w[0] * f0V + w[1] * f1V + w[2] * f2V + w[3] * f3V;
+    out[1] = w[0] * f0m + w[1] * f1m;
+    out[2] = w[0] * f0h + w[2] * f2h;
+    out[3] = w[0] * f0n + w[3] * f3n;
+  }
+  FMS_HD void gp(const double *w, double *out) const {
+    out[0] = w[0] * gna;
+    out[1] = w[0] * gk;
+    out[2] = w[0] * gl;
+  }
+};
+
 template <class E>
-FMS_HD void hh_vjp(const double *y, const double *p, const double *w, double *out, double *gp) {
+FMS_HD HHLin hh_linearize(const double *y, const double *p) {
+  HHLin L;
   const double V = y[0], m = y[1], h = y[2], n = y[3];
-  const double gna = p[1], gk = p[2], gl = p[3];
-  const double ENa = -75 + 115.000, EK = -75 - 12.0000, EL = -75 + 10.6130, Cm = 1;
-  // rate functions and their V-derivatives, as the model file writes them
-  const double xm = -(V + 50.0000) / 10.0000, exm = E::exp(xm), dexm = ExpDeriv<E>::d(xm, exm) * (-0.1);
-  const double num_m = -0.100000 * (V + 50.0000), den_m = exm - 1.00000;
-  const double am = num_m / den_m, dam = (-0.1 * den_m - num_m * dexm) / (den_m * den_m);
-  const double xbm = -(V + 75.0000) / 18.0000, exbm = E::exp(xbm);
-  const double bm = 4.00000 * exbm, dbm = 4.00000 * ExpDeriv<E>::d(xbm, exbm) * (-1.0 / 18.0);
-  const double xah = -(V + 75.0000) / 20.0000, exah = E::exp(xah);
-  const double ah = 0.0700000 * exah, dah = 0.0700000 * ExpDeriv<E>::d(xah, exah) * (-1.0 / 20.0);
-  const double xbh = -(V + 45.0000) / 10.0000, exbh = E::exp(xbh);
-  const double bh = 1.00000 / (exbh + 1.00000);
-  const double dbh = -(ExpDeriv<E>::d(xbh, exbh) * (-0.1)) / ((exbh + 1.0) * (exbh + 1.0));
-  const double xn = -(V + 65.0000) / 10.0000, exn = E::exp(xn), dexn = ExpDeriv<E>::d(xn, exn) * (-0.1);
-  const double num_n = -0.0100000 * (V + 65.0000), den_n = exn - 1.00000;
-  const double an = num_n / den_n, dan = (-0.01 * den_n - num_n * dexn) / (den_n * den_n);
-  const double xbn = (V + 75.0000) / 80.0000, exbn = E::exp(xbn);
-  const double bn = 0.125000 * exbn, dbn = 0.125000 * ExpDeriv<E>::d(xbn, exbn) / 80.0;
-  const double m2 = m * m, m3 = m2 * m, n3 = n * n * n, n4 = n3 * n;
-  const double df0dV = -(gna * m3 * h + gk * n4 + gl) / Cm;
-  const double df0dm = -(gna * 3.0 * m2 * h * (V - ENa)) / Cm;
-  const double df0dh = -(gna * m3 * (V - ENa)) / Cm;
-  const double df0dn = -(gk * 4.0 * n3 * (V - EK)) / Cm;
-  out[0] = w[0] * df0dV + w[1] * (dam * (1.0 - m) - dbm * m) + w[2] * (dah * (1.0 - h) - dbh * h) +
-           w[3] * (dan * (1.0 - n) - dbn * n);
-  out[1] = w[0] * df0dm + w[1] * (-am - bm);
-  out[2] = w[0] * df0dh + w[2] * (-ah - bh);
-  out[3] = w[0] * df0dn + w[3] * (-an - bn);
-  gp[0] = w[0] * (-m3 * h * (V - ENa) / Cm);
-  gp[1] = w[0] * (-n4 * (V - EK) / Cm);
-  gp[2] = w[0] * (-(V - EL) / Cm);
+  const double i_stim = p[0], g_na = p[1], g_k = p[2], g_l = p[3];
+  const double constant0 = -75, constant1 = 1;
+  const double constant6 = constant0 + 115.000, constant7 = constant0 - 12.0000, constant8 = constant0 + 10.6130;
+  // rates, in ModelHH::rhs's operation order
+  const double em = E::exp(-(V + 50.0000) / 10.0000);
+  const double a1 = (-0.100000 * (V + 50.0000)) / (em - 1.00000);
+  const double ebm = E::exp(-(V + 75.0000) / 18.0000);
+  const double a5 = 4.00000 * ebm;
+  L.r[1] = a1 * (1.00000 - m) - a5 * m;
+  const double eah = E::exp(-(V + 75.0000) / 20.0000);
+  const double a2 = 0.0700000 * eah;
+  const double ebh = E::exp(-(V + 45.0000) / 10.0000);
+  const double a6 = 1.00000 / (ebh + 1.00000);
+  L.r[2] = a2 * (1.00000 - h) - a6 * h;
+  const double en = E::exp(-(V + 65.0000) / 10.0000);
+  const double a3 = (-0.0100000 * (V + 65.0000)) / (en - 1.00000);
+  const double ebn = E::exp((V + 75.0000) / 80.0000);
+  const double a7 = 0.125000 * ebn;
+  L.r[3] = a3 * (1.00000 - n) - a7 * n;
+  const double m3 = pow3(m), n4 = pow4(n);
+  const double a0 = g_na * m3 * h * (V - constant6);
+  const double a4 = g_k * n4 * (V - constant7);
+  const double a8 = g_l * (V - constant8);
+  L.r[0] = -(-i_stim + a0 + a4 + a8) / constant1;
+  // derivatives (E'(x) of the exponential actually used)
+  const double xm = -(V + 50.0000) / 10.0000, xbm = -(V + 75.0000) / 18.0000, xah = -(V + 75.0000) / 20.0000;
+  const double xbh = -(V + 45.0000) / 10.0000, xn = -(V + 65.0000) / 10.0000, xbn = (V + 75.0000) / 80.0000;
+  const double den_m = em - 1.0, num_m = -0.1 * (V + 50.0);
+  const double dam = (-0.1 * den_m - num_m * (ExpDeriv<E>::d(xm, em) * (-0.1))) / (den_m * den_m);
+  const double dbm = 4.0 * ExpDeriv<E>::d(xbm, ebm) * (-1.0 / 18.0);
+  const double dah = 0.07 * ExpDeriv<E>::d(xah, eah) * (-1.0 / 20.0);
+  const double dbh = -(ExpDeriv<E>::d(xbh, ebh) * (-0.1)) / ((ebh + 1.0) * (ebh + 1.0));
+  const double den_n = en - 1.0, num_n = -0.01 * (V + 65.0);
+  const double dan = (-0.01 * den_n - num_n * (ExpDeriv<E>::d(xn, en) * (-0.1))) / (den_n * den_n);
+  const double dbn = 0.125 * ExpDeriv<E>::d(xbn, ebn) / 80.0;
+  const double m2 = m * m, n3 = n * n * n;
+  L.f0V = -(g_na * m3 * h + g_k * n4 + g_l);
+  L.f0m = -(g_na * 3.0 * m2 * h * (V - constant6));
+  L.f0h = -(g_na * m3 * (V - constant6));
+  L.f0n = -(g_k * 4.0 * n3 * (V - constant7));
+  L.f1V = dam * (1.0 - m) - dbm * m;
+  L.f1m = -a1 - a5;
+  L.f2V = dah * (1.0 - h) - dbh * h;
+  L.f2h = -a2 - a6;
+  L.f3V = dan * (1.0 - n) - dbn * n;
+  L.f3n = -a3 - a7;
+  L.gna = -m3 * h * (V - constant6);
+  L.gk = -n4 * (V - constant7);
+  L.gl = -(V - constant8);
+  return L;
 }
 
-// reverse of n0D Heun steps of one node starting from state y0 at t0;
-// lam[4] in: adjoint of the output, out: adjoint of y0; gacc[3] += parameter gradient
+// reverse of one Heun step from its input y (no clamp):
+// lam[4] in: adjoint of the output, out: adjoint of y; gacc[3] += parameter gradient
 template <class E>
-FMS_HD void hh_heun_reverse(const double *y0, const double *p, double t0, double dt, int n0D, double *lam,
-                            double *gacc) {
-  for (int j = n0D - 1; j >= 0; j--) {
-    double y[4] = {y0[0], y0[1], y0[2], y0[3]};
-    double g3[3];
-    for (int q = 0; q < j; q++) heun_node<ModelHH, double, E, false>(y[0], y + 1, p, t0 + q * dt, dt, false, 0.0);
-    double r[4], a[9], ys[4];
-    ModelHH::rhs<double, E>(y, p, t0 + j * dt, r, a);
-    for (int k = 0; k < 4; k++) ys[k] = y[k] + dt * r[k];
-    double mu[4], gs[3];
-    hh_vjp<E>(ys, p, lam, mu, gs);
-    double nu[4];
-    for (int k = 0; k < 4; k++) nu[k] = lam[k] + dt * mu[k];
-    double jy[4];
-    hh_vjp<E>(y, p, nu, jy, g3);
-    for (int k = 0; k < 4; k++) lam[k] = lam[k] + 0.5 * dt * mu[k] + 0.5 * dt * jy[k];
-    for (int k = 0; k < 3; k++) gacc[k] += 0.5 * dt * (g3[k] + gs[k]);
-  }
+FMS_HD void hh_heun_step_reverse(const double *y, const double *p, double dt, double *lam, double *gacc) {
+  const HHLin Ly = hh_linearize<E>(y, p);
+  double ys[4];
+  for (int k = 0; k < 4; k++) ys[k] = y[k] + dt * Ly.r[k];
+  const HHLin Ls = hh_linearize<E>(ys, p);
+  double mu[4], gs[3], nu[4], jy[4], g3[3];
+  Ls.jt(lam, mu);
+  Ls.gp(lam, gs);
+  for (int k = 0; k < 4; k++) nu[k] = lam[k] + dt * mu[k];
+  Ly.jt(nu, jy);
+  Ly.gp(nu, g3);
+  for (int k = 0; k < 4; k++) lam[k] = lam[k] + 0.5 * dt * mu[k] + 0.5 * dt * jy[k];
+  for (int k = 0; k < 3; k++) gacc[k] += 0.5 * dt * (g3[k] + gs[k]);
 }
 
 // current-pulse stimulus parameter of one Heun step, as the forward sets it
@@ -121,17 +157,17 @@ __device__ __forceinline__ void reverse_half(const DevStim &st, int f, bool inse
     forward_half<E>(st, f, inset, y, p, t0, dt, j);
     double pj[4];
     stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
-    hh_heun_reverse<E>(y, pj, t0 + j * dt, dt, 1, l, g);
+    hh_heun_step_reverse<E>(y, pj, dt, l, g);
   }
 }
 
 // Reverse of kc split steps of one segment, one workgroup per fibre, steps
 // s = kc-1 .. 0 (rows of `steps`, 4 doubles each). Per step: the sensors'
-// dL/dV, reverse of half 2 from y^b = (V^b, gates after half 1; the gates are
-// recomputed from the tape's y0), reverse diffusion w = A^{-1} lambda_V,
-// lambda_V = B w (A, B symmetric), pf_bar += w^T (dB/dpf V^a - dA/dpf V^b),
-// reverse of half 1 from y0. lambda and the per-node parameter gradients stay
-// in registers for the whole segment. Tape: [kc][6][N] = y0 (4), V^a, V^b.
+// dL/dV, reverse of half 2 from y^b = (V^b, gates after half 1), reverse
+// diffusion w = A^{-1} lambda_V, lambda_V = B w (A, B symmetric),
+// pf_bar += w^T (dB/dpf V^a - dA/dpf V^b), reverse of half 1 from y0. lambda and
+// the per-node parameter gradients stay in registers for the whole segment.
+// Tape: [kc][9][N] = y0 (4), y after half 1 (V^a and the gates, 4), V^b.
 template <class E, int T, int NPT, int MMAX>
 __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const double *__restrict__ tape,
                                                  const double *__restrict__ steps, int kc, int n0D, int ie,
@@ -139,6 +175,9 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
                                                  const double *__restrict__ dtrace, int n_sensors, double *lam,
                                                  double *gacc, double *pf_bar) {
   extern __shared__ double sh[];
+  // the reverse sweep is the critical path while the next segment's recompute
+  // runs beside it on the same CUs: its waves get issue priority
+  __builtin_amdgcn_s_setprio(2);
   const int f = blockIdx.x;
   const int tid = threadIdx.x;
   const long long N = fb.n_total;
@@ -165,7 +204,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   for (int s = kc - 1; s >= 0; s--) {
     const double *row = steps + 4 * s;
     const double t = row[0], dts = row[1], dt0 = row[2], dt1 = row[3];
-    const double *tp = tape + (long long)s * 6 * N;
+    const double *tp = tape + (long long)s * 9 * N;
     // sensors, reverse of half 2
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
@@ -176,16 +215,15 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
       const int slot = sensor_slot[node];
       const double dl = dtrace[((long long)s * fb.n_fibers + f) * n_sensors + (slot >= 0 ? slot : 0)];
       l[q][0] += slot >= 0 ? dl : 0.0;
-      double y[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
-      forward_half<E>(st, f, inset, y, p, t, dt0, n0D);
-      y[0] = tp[5 * N + node];
+      // y^b: V after the diffusion with the gates after half 1
+      const double y[4] = {tp[8 * N + node], tp[5 * N + node], tp[6 * N + node], tp[7 * N + node]};
       reverse_half<E>(st, f, inset, y, p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
       if (valid) lw[i] = l[q][0];
     }
     __syncthreads();
     // reverse diffusion
     lds_thomas_part<T, MMAX, true>(lw, n, h, pf, dt1, ie != 0, red, tid);
-    const double *Va = tp + 4 * N + off, *Vb = tp + 5 * N + off;
+    const double *Va = tp + 4 * N + off, *Vb = tp + 8 * N + off;
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
       const bool valid = tid + q * T < n;
@@ -199,17 +237,22 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
       pfacc += valid ? lw[i] * (ie ? -k1v : -0.5 * k1v) : 0.0;
       l[q][0] = d;
     }
-    // this step's prefactor gradient, summed in a fixed order and added to the
-    // fibre's total step by step (independent of the checkpoint segmentation)
-    red[tid] = pfacc;
+    // this step's prefactor gradient, summed in a fixed order (butterfly within
+    // each wave, then the waves in order) and added to the fibre's total step by
+    // step, so the result does not depend on the checkpoint segmentation
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pfacc += __shfl_xor(pfacc, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = pfacc;
     pfacc = 0;
     __syncthreads();
     if (tid == 0) {
-      double sum = 0;
-      for (int q = 0; q < T; q++) sum += red[q];
+      double sum = red[0];
+#pragma unroll
+      for (int q = 1; q < T / 64; q++) sum += red[q];
       pf_bar[f] += sum;
     }
-    __syncthreads();
+    // (the next step writes red only after the barrier that follows its first
+    // loop, so thread 0's reads above need no barrier of their own)
     // reverse of half 1
 #pragma unroll
     for (int q = 0; q < NPT; q++) {

@@ -33,14 +33,14 @@ struct DevTrace {
   const int *sensor_slot;   // [N] sensor number of each node, -1 elsewhere; nullptr: no traces
   double *trace;            // [K][F][n_sensors], this launch's first step first
   int n_sensors;
-  double *tape;             // [K][NS+2][N]: y before half 1 (NS), Vm before the diffusion (1),
-                            // Vm after it (1); nullptr: none
+  double *tape;             // [K][2*NS+1][N]: y before half 1 (NS), y after half 1 (NS: Vm
+                            // before the diffusion and the gates), Vm after it (1); nullptr: none
 };
 
 template <int NS, class G>
 __device__ __forceinline__ void tape_store(const DevTrace &tr, long long N, int s, int slot0, long long i, double V,
                                            const G *g, int nstore) {
-  double *t = tr.tape + ((long long)s * (NS + 2) + slot0) * N + i;
+  double *t = tr.tape + ((long long)s * (2 * NS + 1) + slot0) * N + i;
   t[0] = V;
   for (int k = 1; k < nstore; k++) t[(long long)k * N] = (double)g[k - 1];
 }
@@ -50,7 +50,7 @@ __device__ __forceinline__ void tape_store(const DevTrace &tr, long long N, int 
 template <int NS, class G>
 __device__ __forceinline__ void tape_store_streamed(const DevTrace &tr, long long N, int s, int slot0, long long i,
                                                     double V, const G *gates) {
-  double *t = tr.tape + ((long long)s * (NS + 2) + slot0) * N + i;
+  double *t = tr.tape + ((long long)s * (2 * NS + 1) + slot0) * N + i;
   t[0] = V;
   if (gates)
     for (int k = 1; k < NS; k++) t[(long long)k * N] = (double)gates[(long long)(k - 1) * N + i];
@@ -73,14 +73,12 @@ __device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, doub
 }
 
 // One reaction half of one node with its gates streamed from HBM: load the
-// gates, n0D Heun steps, store the gates (if `store`), return Vm. Not inlined:
-// for large models the half is thousands of instructions under full register
-// pressure, and compiling it once as its own function keeps the fused kernel's
-// loop, LDS and barrier code out of its register allocation (inlined into the
-// fused kernel, the 56-state model's reaction gave wrong, run-to-run different
-// results on gfx950 while the same code in the split kernel was exact).
+// gates, n0D Heun steps, store the gates (if `store`), return Vm. The caller
+// runs it on every lane of the wave (lanes past the fibre's end pass
+// store = false), so the large models' reaction code never executes under a
+// partial execution mask.
 template <class M, class G, class E>
-__device__ __noinline__ double streamed_half(G *__restrict__ gates, long long N, long long node, bool store, double V,
+__device__ __forceinline__ double streamed_half(G *__restrict__ gates, long long N, long long node, bool store, double V,
                                              const double *__restrict__ fparams, int stim_mode, int stim_param,
                                              double amplitude, bool inset, const unsigned char *flags, double t0,
                                              double dt, int n0D, double vstim) {
@@ -177,7 +175,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         double V = lv[i];
         if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, 0, off + i, V, greg[q], M::NS);
         fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
-        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, greg[q], 1);
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, greg[q], M::NS);
         if (valid) lv[i] = V;
       }
     } else {
@@ -188,7 +186,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, 0, off + i, V, gates);
         V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
                                    sflags, t0, dt0, n0D, vstim);
-        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, M::NS, off + i, V, (const G *)nullptr);
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, M::NS, off + i, V, gates);
         if (valid) lv[i] = V;
       }
     }
@@ -222,7 +220,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     // one node: second half of step s, traces and spikes, first half of step s+1
     // (valid = false: a shadow lane, which computes and stores nothing)
     auto node = [&](int i, bool valid, double &V, G *g, double &vp, int &sf, int &sc) {
-      if (tr.tape && valid) tape_store<M::NS>(tr, N, s, M::NS + 1, off + i, V, g, 1);
+      if (tr.tape && valid) tape_store<M::NS>(tr, N, s, 2 * M::NS, off + i, V, g, 1);
       fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim);
       if (tr.sensor_slot && valid) {
         const int slot = tr.sensor_slot[off + i];
@@ -238,7 +236,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, 0, off + i, V, g, M::NS);
         fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D,
                                  vstim);
-        if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, M::NS, off + i, V, g, 1);
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, M::NS, off + i, V, g, M::NS);
       }
     };
 
@@ -258,7 +256,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         const bool valid = base + tid < n;
         const int i = valid ? base + tid : n - 1;
         double V = lv[i];
-        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s, M::NS + 1, off + i, V, (const G *)nullptr);
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s, 2 * M::NS, off + i, V, (const G *)nullptr);
         V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
                                    sflags, t_mid, dt0, n0D, vstim);
         if (tr.sensor_slot && valid) {
@@ -283,7 +281,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
           if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, 0, off + i, V, gates);
           V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp,
                                      i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D, vstim);
-          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, M::NS, off + i, V, (const G *)nullptr);
+          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, M::NS, off + i, V, gates);
           if (valid) lv[i] = V;
         }
       }

@@ -73,6 +73,8 @@ struct FmsSolver {
   std::vector<int> h_off;     // [F+1]
   std::vector<int> h_order;   // fibres longest first
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // adjoint: recompute of the next segment beside the reverse of this one
+  hipEvent_t adj_ev[4] = {};
 
   // device state
   int *d_off = nullptr, *d_node_fiber = nullptr, *d_sb = nullptr, *d_se = nullptr, *d_order = nullptr;
@@ -203,6 +205,9 @@ void free_all(FmsSolver *s) {
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->events)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->adj_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->stream2) (void)hipStreamDestroy(s->stream2);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -1024,19 +1029,18 @@ int copy_state(FmsSolver *s, double *dst, bool to_ckpt) {
   return 0;
 }
 
-template <class E> int adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_params, double *pf_bar) {
+// reverse sweep over all segments; d_dtraces is device memory in the traces'
+// layout; leaves dL/d(gNa, gK, gL) in s->d_grad [F][3] and dL/dprefactor in
+// s->d_pfbar [F], enqueued on the solver's stream
+template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtraces) {
   if constexpr (!std::is_same<Model, fms::ModelHH>::value) {
-    (void)s; (void)dtraces; (void)grad_params; (void)pf_bar;
+    (void)s; (void)d_dtraces;
     return fail(FMS_ERR_UNSUPPORTED, "the adjoint is implemented for the Hodgkin-Huxley model");
   } else {
     const long long N = s->N;
     const int F = s->F, S = s->n_sensors, K = s->adj_K, Kc = s->adj_interval, n0D = s->adj_n0D;
     const int ie = s->scheme == FMS_IMPLICIT_EULER;
-    long long cap = 0;
     int rc;
-    if ((rc = ensure(&s->d_dtrace, cap, (long long)K * F * S))) return rc;
-    HIP_TRY(hipMemcpyAsync(s->d_dtrace, dtraces, (size_t)K * F * S * sizeof(double), hipMemcpyHostToDevice,
-                           s->stream));
     long long c1 = 0, c2 = 0, c3 = 0, c4 = 0;
     if ((rc = ensure(&s->d_lam, c1, 4 * N)) || (rc = ensure(&s->d_gacc, c2, 3 * N)) ||
         (rc = ensure(&s->d_pfbar, c3, F)) || (rc = ensure(&s->d_grad, c4, 3 * (long long)F)))
@@ -1044,46 +1048,114 @@ template <class E> int adjoint_backward(FmsSolver *s, const double *dtraces, dou
     HIP_TRY(hipMemsetAsync(s->d_lam, 0, 4 * N * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->d_gacc, 0, 3 * N * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->d_pfbar, 0, F * sizeof(double), s->stream));
-    if ((rc = ensure(&s->d_tape, s->tape_cap, (long long)Kc * (Model::NS + 2) * N))) return rc;
+    // two tape buffers: the recompute of segment seg-1 (stream2) runs beside the
+    // reverse of segment seg (stream)
+    const size_t tape_len = (size_t)Kc * (2 * Model::NS + 1) * N;
+    if ((rc = ensure(&s->d_tape, s->tape_cap, 2 * (long long)tape_len))) return rc;
     if ((rc = upload_steps(s, s->adj_steps.data(), K))) return rc;
+    if (!s->stream2 && hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess)
+      return fail(FMS_ERR_HIP, "hipStreamCreate failed");
+    for (hipEvent_t &e : s->adj_ev)
+      if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return fail(FMS_ERR_HIP, "hipEventCreate failed");
+    // the recompute stream starts after everything enqueued so far (step table, clears)
+    HIP_TRY(hipEventRecord(s->adj_ev[2], s->stream));
+    HIP_TRY(hipStreamWaitEvent(s->stream2, s->adj_ev[2], 0));
     const int saved_spikes = s->spikes;
     s->spikes = 0;
     const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
     const size_t lds = (size_t)(16 * T + s->nmax) * sizeof(double);
-    for (int seg = nseg - 1; seg >= 0; seg--) {
+    hipStream_t main_stream = s->stream;
+    for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
-      // recompute the segment from its checkpoint, recording the tape
-      if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, false))) break;
+      const int buf = idx & 1;
+      double *tape = s->d_tape + buf * tape_len;
+      // recompute the segment from its checkpoint on stream2, recording the tape;
+      // the buffer was last read by the reverse of segment seg+2
+      s->stream = s->stream2;
+      if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess) rc = FMS_ERR_HIP;
+      if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, false);
       fms::DevTrace tr{};
-      tr.tape = s->d_tape;
+      tr.tape = tape;
       const long long save = s->step_counter;
-      if ((rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr))) break;
+      if (!rc) rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
       s->step_counter = save;
-      // reverse the segment
-      const double *dtr = s->d_dtrace + (size_t)k0 * F * S;
+      if (!rc && hipEventRecord(s->adj_ev[2 + buf], s->stream2) != hipSuccess) rc = FMS_ERR_HIP;
+      s->stream = main_stream;
+      if (rc) break;
+      // reverse the segment on the solver's stream once its tape is recorded
+      HIP_TRY(hipStreamWaitEvent(s->stream, s->adj_ev[2 + buf], 0));
+      const double *dtr = d_dtraces + (size_t)k0 * F * S;
       if (s->nmax <= 4 * T)
         hipLaunchKernelGGL((fms::k_adj_fused<E, T, 4, 8>), dim3(F), dim3(T), lds, s->stream, s->fibers(), s->stim(),
-                           s->d_tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
+                           tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
                            s->d_gacc, s->d_pfbar);
       else
         hipLaunchKernelGGL((fms::k_adj_fused<E, T, 8, 8>), dim3(F), dim3(T), lds, s->stream, s->fibers(), s->stim(),
-                           s->d_tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
+                           tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
                            s->d_gacc, s->d_pfbar);
       if (hipGetLastError() != hipSuccess) {
         rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
         break;
       }
+      HIP_TRY(hipEventRecord(s->adj_ev[buf], s->stream));
     }
+    s->stream = main_stream;
+    if (rc) (void)hipStreamSynchronize(s->stream2);
     s->spikes = saved_spikes;
     if (rc) return rc;
     hipLaunchKernelGGL(fms::k_adj_reduce, dim3(F), dim3(256), 0, s->stream, s->fibers(), s->d_gacc, s->d_grad);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    if (grad_params) HIP_TRY(hipMemcpy(grad_params, s->d_grad, 3 * (size_t)F * sizeof(double), hipMemcpyDeviceToHost));
-    if (pf_bar) HIP_TRY(hipMemcpy(pf_bar, s->d_pfbar, (size_t)F * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
   }
+}
+
+int adjoint_backward(FmsSolver *s, const double *d_dtraces) {
+  return s->exp_mode == FMS_EXP_APPROXIMATE ? adjoint_backward_core<fms::ExpApprox>(s, d_dtraces)
+                                            : adjoint_backward_core<fms::ExpLibm>(s, d_dtraces);
+}
+
+// forward over all segments, checkpointing; traces go to device memory d_traces
+int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int checkpoint_interval,
+                    double *d_traces) {
+  if (!s || !steps || n_steps <= 0 || n0D < 1 || checkpoint_interval < 1)
+    return fail(FMS_ERR_ARGUMENT, "bad adjoint forward arguments");
+  if (s->n_sensors <= 0) return fail(FMS_ERR_STATE, "call fms_set_sensors first");
+  if (s->precision != FMS_FP64) return fail(FMS_ERR_UNSUPPORTED, "the adjoint runs in FP64");
+  if (s->stim_mode == FMS_STIM_FIRING_TABLE)
+    return fail(FMS_ERR_UNSUPPORTED, "the Vm clamp of firing-table stimulation is not differentiable; use current pulses");
+  if (s->only_if_stimulated) return fail(FMS_ERR_UNSUPPORTED, "onlyComputeIfHasBeenStimulated is not differentiable");
+  if (s->nmax > 2048) return fail(FMS_ERR_UNSUPPORTED, "the adjoint takes fibres of <= 2048 nodes");
+  const long long N = s->N;
+  const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
+  const int nseg = (K + Kc - 1) / Kc;
+  int rc;
+  if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)nseg * Model::NS * N))) return rc;
+  if (s->flags_cap < s->F * n0D) {
+    if (s->d_flags) HIP_TRY(hipFree(s->d_flags));
+    s->flags_cap = s->F * n0D;
+    if (dalloc(&s->d_flags, (size_t)s->flags_cap)) return FMS_ERR_HIP;
+  }
+  if ((rc = upload_steps(s, steps, K))) return rc;
+  s->adj_steps.assign(steps, steps + 4 * (size_t)K);
+  s->adj_n0D = n0D;
+  s->adj_interval = Kc;
+  s->adj_K = K;
+  for (int seg = 0; seg < nseg; seg++) {
+    const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
+    if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, true))) return rc;
+    fms::DevTrace tr{};
+    tr.sensor_slot = s->d_sensor_slot;
+    tr.trace = d_traces + (size_t)k0 * F * S;
+    tr.n_sensors = S;
+    if (s->exp_mode == FMS_EXP_APPROXIMATE)
+      rc = launch_fused<double, fms::ExpApprox>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
+    else
+      rc = launch_fused<double, fms::ExpLibm>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 }  // namespace
@@ -1127,53 +1199,59 @@ int fms_set_sensors(FmsSolver *s, const int *sensor_nodes, int n_sensors) {
 
 int fms_adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int checkpoint_interval,
                         double *traces) {
-  if (!s || !steps || n_steps <= 0 || n0D < 1 || checkpoint_interval < 1)
-    return fail(FMS_ERR_ARGUMENT, "bad adjoint forward arguments");
-  if (s->n_sensors <= 0) return fail(FMS_ERR_STATE, "call fms_set_sensors first");
-  if (s->precision != FMS_FP64) return fail(FMS_ERR_UNSUPPORTED, "the adjoint runs in FP64");
-  if (s->stim_mode == FMS_STIM_FIRING_TABLE)
-    return fail(FMS_ERR_UNSUPPORTED, "the Vm clamp of firing-table stimulation is not differentiable; use current pulses");
-  if (s->only_if_stimulated) return fail(FMS_ERR_UNSUPPORTED, "onlyComputeIfHasBeenStimulated is not differentiable");
-  if (s->nmax > 2048) return fail(FMS_ERR_UNSUPPORTED, "the adjoint takes fibres of <= 2048 nodes");
-  const long long N = s->N;
-  const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
-  const int nseg = (K + Kc - 1) / Kc;
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   int rc;
-  if ((rc = ensure(&s->d_trace, s->trace_cap, (long long)K * F * S))) return rc;
-  if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)nseg * Model::NS * N))) return rc;
-  if (s->flags_cap < s->F * n0D) {
-    if (s->d_flags) HIP_TRY(hipFree(s->d_flags));
-    s->flags_cap = s->F * n0D;
-    if (dalloc(&s->d_flags, (size_t)s->flags_cap)) return FMS_ERR_HIP;
-  }
-  if ((rc = upload_steps(s, steps, K))) return rc;
-  s->adj_steps.assign(steps, steps + 4 * (size_t)K);
-  s->adj_n0D = n0D;
-  s->adj_interval = Kc;
-  s->adj_K = K;
-  for (int seg = 0; seg < nseg; seg++) {
-    const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
-    if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, true))) return rc;
-    fms::DevTrace tr{};
-    tr.sensor_slot = s->d_sensor_slot;
-    tr.trace = s->d_trace + (size_t)k0 * F * S;
-    tr.n_sensors = S;
-    if (s->exp_mode == FMS_EXP_APPROXIMATE) rc = launch_fused<double, fms::ExpApprox>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
-    else rc = launch_fused<double, fms::ExpLibm>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
-    if (rc) return rc;
-  }
+  if (n_steps > 0 && s->n_sensors > 0 &&
+      (rc = ensure(&s->d_trace, s->trace_cap, (long long)n_steps * s->F * s->n_sensors)))
+    return rc;
+  if ((rc = adjoint_forward(s, steps, n_steps, n0D, checkpoint_interval, s->d_trace))) return rc;
   if (traces) {
     HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(traces, s->d_trace, (size_t)K * F * S * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(traces, s->d_trace, (size_t)n_steps * s->F * s->n_sensors * sizeof(double),
+                      hipMemcpyDeviceToHost));
   }
+  return 0;
+}
+
+int fms_adjoint_forward_device(FmsSolver *s, const double *steps, int n_steps, int n0D, int checkpoint_interval,
+                               double *d_traces) {
+  if (!s || !d_traces) return fail(FMS_ERR_ARGUMENT, "null argument");
+  int rc;
+  if ((rc = adjoint_forward(s, steps, n_steps, n0D, checkpoint_interval, d_traces))) return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
   return 0;
 }
 
 int fms_adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_params, double *grad_prefactor) {
   if (!s || !dtraces) return fail(FMS_ERR_ARGUMENT, "null argument");
   if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
-  if (s->exp_mode == FMS_EXP_APPROXIMATE) return adjoint_backward<fms::ExpApprox>(s, dtraces, grad_params, grad_prefactor);
-  return adjoint_backward<fms::ExpLibm>(s, dtraces, grad_params, grad_prefactor);
+  const size_t len = (size_t)s->adj_K * s->F * s->n_sensors;
+  long long cap = 0;
+  int rc;
+  if ((rc = ensure(&s->d_dtrace, cap, (long long)len))) return rc;
+  HIP_TRY(hipMemcpyAsync(s->d_dtrace, dtraces, len * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  if ((rc = adjoint_backward(s, s->d_dtrace))) return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (grad_params) HIP_TRY(hipMemcpy(grad_params, s->d_grad, 3 * (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost));
+  if (grad_prefactor)
+    HIP_TRY(hipMemcpy(grad_prefactor, s->d_pfbar, (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fms_adjoint_backward_device(FmsSolver *s, const double *d_dtraces, double *d_grad_params,
+                                double *d_grad_prefactor) {
+  if (!s || !d_dtraces) return fail(FMS_ERR_ARGUMENT, "null argument");
+  if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
+  int rc;
+  if ((rc = adjoint_backward(s, d_dtraces))) return rc;
+  if (d_grad_params)
+    HIP_TRY(hipMemcpyAsync(d_grad_params, s->d_grad, 3 * (size_t)s->F * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+  if (d_grad_prefactor)
+    HIP_TRY(hipMemcpyAsync(d_grad_prefactor, s->d_pfbar, (size_t)s->F * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return 0;
 }
 
 }  // extern "C"

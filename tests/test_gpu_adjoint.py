@@ -74,10 +74,11 @@ def test_adjoint_matches_central_differences(scheme):
 
 @pytest.mark.parametrize("scheme", ["CN", "IE"])
 def test_adjoint_exact_on_a_short_horizon(scheme):
-    """80 steps: the loss is smooth in theta, so Richardson-extrapolated central
-    differences are accurate to ~1e-9 and the discrete adjoint must agree with
-    them to 1e-7 in random directions. The target is offset by 1 mV so that the
-    residuals (and the loss differences) are far above the traces' rounding."""
+    """80 steps, target offset by 1 mV: the adjoint against Richardson-extrapolated
+    central differences (h = 1e-5, 2e-5) in random directions. The differences'
+    own error here is ~1e-7 (the dense numpy adjoint of the same problem shows
+    the same gap to its differences), so the bar is 1e-6; the exact check is
+    test_adjoint_equals_dense_reference."""
     prob, theta_true, theta0 = _problem(scheme=scheme, n_steps=80, interval=30)
     target = prob.forward(theta_true) + 1.0
     L0, tr = _loss(prob, theta0, target)
@@ -93,7 +94,7 @@ def test_adjoint_exact_on_a_short_horizon(scheme):
             d.append((lp - lm) / (2 * hq))
         dr = (4 * d[0] - d[1]) / 3
         print(f"{scheme} direction {trial}: adjoint {ad:.12e} Richardson FD {dr:.12e} rel {abs(ad - dr) / abs(dr):.2e}")
-        assert abs(ad - dr) <= 1e-7 * abs(dr)
+        assert abs(ad - dr) <= 1e-6 * abs(dr)
 
 
 def test_adjoint_traces_match_forward_run():
@@ -166,3 +167,20 @@ def test_adjoint_equals_dense_reference(scheme):
         rg = R.gradient(fib, theta[f], steps, scheme == "IE", dtr[:, f, :], AM * CM)
         print(f"{scheme} fibre {f}: device {grad[f]} reference {rg} rel {np.abs(grad[f] - rg) / np.abs(rg)}")
         np.testing.assert_allclose(grad[f], rg, rtol=1e-9)
+
+
+def test_torch_autograd_on_device_equals_host_path():
+    torch = pytest.importorskip("torch")
+    from opendihu_amd.adjoint import MonodomainTraces
+    prob, theta_true, theta0 = _problem(n_steps=300, interval=64)
+    dev = torch.device("cuda")
+    target = prob.forward_tensor(torch.tensor(theta_true, device=dev))
+    th = torch.tensor(theta0, dtype=torch.float64, device=dev, requires_grad=True)
+    tr = MonodomainTraces.apply(th, prob)
+    assert tr.is_cuda
+    loss = 0.5 * ((tr - target) ** 2).sum()
+    loss.backward()
+    tr_host = prob.forward(theta0)
+    np.testing.assert_array_equal(tr.detach().cpu().numpy(), tr_host)
+    expect = prob.backward(tr_host - target.cpu().numpy())
+    np.testing.assert_allclose(th.grad.cpu().numpy(), expect, rtol=1e-12, atol=0)
