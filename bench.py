@@ -44,9 +44,10 @@ FP64_PEAK_TFLOPS = 78.6
 
 SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 
-# HBM bytes per fiber-node-step of k_fused measured with rocprofv3 (FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000, factors from a 1 GiB copy), profiles/k_fused_shorten56_1024x1481.md
-MEASURED_TRAFFIC_PER_NODE_STEP = {("c2", "fused"): 941.9}
+# HBM bytes per node and launch unit (reaction: one half step; fused: one split
+# step) of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000, factors from a 1 GiB copy); the summaries are in profiles/
+MEASURED_TRAFFIC_PER_NODE = {}
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
@@ -56,7 +57,12 @@ def bytes_per_node_step(n_states: int, precision: str) -> int:
     return 2 * 8 + 2 * 4 * (n_states - 1)
 
 
+def step_table(t0, dt, n):
+    return np.stack([t0 + np.arange(n) * dt, np.full(n, dt), np.full(n, dt), np.full(n, dt)], 1)
+
+
 def make_workload(name: str, precision: str, kernels: str):
+    kernels = "fused" if kernels == "auto" else kernels
     from cases import c2, uneven, c1
     from opendihu_amd.fibers import FiberBatch
     from opendihu_amd.build import SHORTEN_LIBRARY, HH_LIBRARY
@@ -264,6 +270,7 @@ def bench_c5(steps=200, warmup=20):
                        library=SHORTEN_LIBRARY)
         g.set_firing_table_stimulation(**stim)
         g.enable_spike_detection(0.0)
+        sched = g.autotune(step_table(0.0, dt, 8), allow_split=True)
         g.advance(0.0, dt, warmup)
         g.synchronize()
         w0 = time.perf_counter()
@@ -271,12 +278,13 @@ def bench_c5(steps=200, warmup=20):
         g.synchronize()
         wall = time.perf_counter() - w0
         res[precision] = dict(value=g.n_total * steps / wall, ms_per_step=wall / steps * 1e3, vm=g.vm(),
-                              first=g.spikes()[0])
+                              first=g.spikes()[0], schedule=sched)
         g.close()
     spikes = _spike_shift(res["mixed"]["first"], res["fp64"]["first"], steps + warmup)
     return dict(workload="shorten56_16384_uneven_1000-1481_IE_dt2.5e-5_100MU", nodes=int(mesh["n_nodes"].sum()),
                 steps=steps, warmup=warmup, value_mixed=res["mixed"]["value"], value_fp64=res["fp64"]["value"],
                 ms_per_step_mixed=res["mixed"]["ms_per_step"], ms_per_step_fp64=res["fp64"]["ms_per_step"],
+                schedule_mixed=res["mixed"]["schedule"], schedule_fp64=res["fp64"]["schedule"],
                 mixed_vs_fp64=dict(spikes, vm_max_abs_err_mV=float(np.max(np.abs(res["mixed"]["vm"] - res["fp64"]["vm"]))),
                                    after_steps=steps + warmup))
 
@@ -287,7 +295,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c1"])
-    ap.add_argument("--kernels", default="fused", choices=["fused", "split"])
+    ap.add_argument("--kernels", default="auto", choices=["auto", "fused", "split"],
+                    help="auto: the autotuner's fastest bit-identical schedule for this batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -313,28 +322,54 @@ def main(argv=None):
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}))
         return 0
 
+    from opendihu_amd import _lib as L
     g, mesh, stim, desc, dt, scheme, oracle_model = make_workload(args.workload, "fp64", args.kernels)
+    schedule = None
+    if args.kernels == "auto":
+        schedule = g.autotune(step_table(0.0, dt, 16), allow_split=True)
     wall, ev_ms, launches = time_gpu(g, dt, args.steps, args.warmup)
     n_total = g.n_total
     value = n_total * args.steps / wall
-    from opendihu_amd import _lib as L
-    b_alg = bytes_per_node_step(g.n_states, "fp64")
-    f_alg = L.flops_per_node_step(g.lib, g.n_states, 1, scheme)
-    kernel = "k_fused" if args.kernels == "fused" else "k_reaction"
-    n_launch = launches["fused"] if args.kernels == "fused" else launches["reaction"]
-    mean_launch_ms = ev_ms / max(1, n_launch)
-    steps_per_launch = args.steps / max(1, n_launch)
-    node_steps_per_launch = n_total * (steps_per_launch if args.kernels == "fused" else 0.5)
-    achieved_gbs = b_alg * node_steps_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    achieved_tflops = f_alg * node_steps_per_launch / (mean_launch_ms * 1e-3) / 1e12
+    # dominant kernel: launch durations from HIP events around each launch on the
+    # solver's stream, in a separate pass after the timed one
+    g.profile_kernels(True)
+    g.set_tuning(graph_steps=0)   # graph replays are not timed per launch; the kernels are the same
+    g.kernel_times(reset=True)
+    t_prof = (args.warmup + args.steps) * dt
+    g.advance(t_prof, dt, 64)
+    g.synchronize()
+    times = g.kernel_times()
+    g.profile_kernels(False)
+    kclass = max(("reaction", "fused"), key=lambda k: times[k][0])
+    total_ms, n_launch = times[kclass]
+    mean_launch_ms = total_ms / max(1, n_launch)
+    ops = L.model_ops(g.lib)["ops_rhs"]
+    if kclass == "reaction":
+        # one launch = one reaction half (n0D = 1 Heun step) over all nodes
+        per_fibre = (schedule or {}).get("reaction_per_fibre", 1)
+        kernel = "k_reaction_fibre" if g.n_states > 8 and per_fibre else "k_reaction"
+        units_per_launch = n_total
+        f_unit = 2 * ops + 5 * g.n_states
+        b_unit = bytes_per_node_step(g.n_states, "fp64") // 1
+    else:
+        kernel = "k_fused"
+        units_per_launch = n_total * 64 / max(1, n_launch)
+        f_unit = L.flops_per_node_step(g.lib, g.n_states, 1, scheme)
+        b_unit = bytes_per_node_step(g.n_states, "fp64")
+    achieved_tflops = f_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e12
+    achieved_gbs = b_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    traffic_per_node = MEASURED_TRAFFIC_PER_NODE.get((args.workload, kclass))
+    kernel_times = {k: dict(total_ms=v[0], launches=v[1]) for k, v in times.items()}
     g.close()
 
     mixed = None
     if not args.no_mixed:
         gm, *_ = make_workload(args.workload, "mixed", args.kernels)
+        mixed_schedule = gm.autotune(step_table(0.0, dt, 16), allow_split=True) if args.kernels == "auto" else None
         wall_m, ev_m, _ = time_gpu(gm, dt, args.steps, args.warmup)
         mixed = dict(value=n_total * args.steps / wall_m, ms_per_step=wall_m / args.steps * 1e3,
-                     bytes_per_node_step=bytes_per_node_step(gm.n_states, "mixed"), dtype="f32 states + f64 Vm/Thomas")
+                     bytes_per_node_step=bytes_per_node_step(gm.n_states, "mixed"), dtype="f32 states + f64 Vm/Thomas",
+                     schedule=mixed_schedule)
         gm.close()
 
     cpu = None
@@ -369,7 +404,8 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded fibre meshes, motor units and firing times; no checkpoints)",
-        "config": dict(desc, kernels=args.kernels, thomas="partitioned", exp="approximate (1+x/1024)^1024"),
+        "config": dict(desc, kernels=args.kernels, schedule=schedule, thomas="partitioned",
+                       exp="approximate (1+x/1024)^1024"),
         "roofline": {
             "bound": "valu",
             "kernel": kernel,
@@ -377,13 +413,14 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": (MEASURED_TRAFFIC_PER_NODE_STEP[(args.workload, args.kernels)] * node_steps_per_launch
-                        if (args.workload, args.kernels) in MEASURED_TRAFFIC_PER_NODE_STEP else None),
-            "flops_per_node_step": f_alg,
-            "launches": n_launch,
+            "traffic": traffic_per_node * units_per_launch if traffic_per_node else None,
+            "ops_per_node_and_launch_unit": f_unit,
+            "nodes_x_units_per_launch": units_per_launch,
             "mean_launch_ms": mean_launch_ms,
+            "launches_timed": n_launch,
+            "kernel_times_64_steps": kernel_times,
             "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_node_step": b_alg},
+                    "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_node_and_launch_unit": b_unit},
         },
         "cpu_baseline": cpu,
         "mixed": mixed,

@@ -160,7 +160,20 @@ int fms_last_error(char *buf, int len);
 #define FMS_TUNE_ONLY_IF_STIMULATED 3
 #define FMS_TUNE_THREADS 4            /* threads per fibre workgroup of the fused kernel: 256 (default) or 128 */
 #define FMS_TUNE_GRAPH_STEPS 5        /* split kernels: steps per captured hipGraph, 0 = launch eagerly (default) */
+#define FMS_TUNE_REACTION_PER_FIBRE 6 /* split kernels, models of > 8 states: reaction with one workgroup per
+                                         fibre (1, default) or one lane per node over the whole batch (0) */
 int fms_set_tuning(FmsSolver *solver, int key, int value);
+
+#define FMS_TUNE_PROFILE_KERNELS 7    /* 1: time every launch with HIP events (serialises the host with the GPU) */
+
+/* accumulated launch durations in ms and launch counts per kernel class while
+   FMS_TUNE_PROFILE_KERNELS is on (graph replays are not timed); reset != 0
+   clears them */
+#define FMS_KERNEL_REACTION 0
+#define FMS_KERNEL_DIFFUSION 1
+#define FMS_KERNEL_FUSED 2
+#define FMS_KERNEL_OTHER 3
+int fms_kernel_times(FmsSolver *solver, double ms[4], long long launches[4], int reset);
 
 /* graphs captured and graph replays issued so far (split kernels) */
 int fms_graph_counts(FmsSolver *solver, long long *captures, long long *replays);
@@ -169,7 +182,8 @@ int fms_graph_counts(FmsSolver *solver, long long *captures, long long *replays)
    and an OpenMP-offload loop): times every schedule that gives bit-identical
    states on this batch - fused kernel with gates resident or streamed, 128 or
    256 threads per fibre, 64 or 512 steps per launch; with allow_split also the
-   split kernels launched eagerly or replayed as hipGraphs of 8 or 32 steps -
+   split kernels launched eagerly or replayed as hipGraphs of 8 or 32 steps
+   (models of > 8 states: reaction per fibre or per node, eager or graphs of 8) -
    over the n_steps records of `steps` (4 doubles each, as
    fms_advance_schedule), restores the states, stimulation state and spike
    counters after every trial and keeps the fastest schedule. candidate_ms
@@ -183,6 +197,7 @@ typedef struct {
   int graph_steps;
   double ms_per_step;
   int n_candidates;
+  int reaction_per_fibre;
 } FmsTuneResult;
 int fms_autotune(FmsSolver *solver, const double *steps, int n_steps, int n0D, int allow_split,
                  FmsTuneResult *result, double *candidate_ms, int capacity);

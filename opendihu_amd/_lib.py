@@ -19,7 +19,8 @@ FMS_THOMAS_SERIAL, FMS_THOMAS_PARTITIONED = 0, 1
 FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
 FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
 FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES, FMS_TUNE_ONLY_IF_STIMULATED = 1, 2, 3
-FMS_TUNE_THREADS, FMS_TUNE_GRAPH_STEPS = 4, 5
+FMS_TUNE_THREADS, FMS_TUNE_GRAPH_STEPS, FMS_TUNE_REACTION_PER_FIBRE, FMS_TUNE_PROFILE_KERNELS = 4, 5, 6, 7
+FMS_KERNEL_CLASSES = ("reaction", "diffusion", "fused", "other")
 
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -35,6 +36,7 @@ class FmsTuneResult(ctypes.Structure):
         ("graph_steps", ctypes.c_int),
         ("ms_per_step", ctypes.c_double),
         ("n_candidates", ctypes.c_int),
+        ("reaction_per_fibre", ctypes.c_int),
     ]
 
 
@@ -105,6 +107,7 @@ EXPORTED_SYMBOLS = [
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
     "fms_eval_rhs", "fms_eval_heun", "fms_adjoint_forward_device", "fms_adjoint_backward_device",
+    "fms_kernel_times",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -169,6 +172,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_adjoint_backward_device.argtypes = [v, v, v, v]
     lib.fms_eval_rhs.argtypes = [_dp, _dp, d, i, i, _dp, _dp]
     lib.fms_eval_heun.argtypes = [_dp, _dp, d, d, i, i, i]
+    lib.fms_kernel_times.argtypes = [v, _dp, ctypes.POINTER(ctypes.c_longlong), i]
     lib.fms_graph_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     lib.fms_autotune.argtypes = [v, _dp, i, i, i, ctypes.POINTER(FmsTuneResult), _dp, i]
     lib.fms_reference_set_dims.argtypes = [ctypes.POINTER(FmsReferenceDims)]

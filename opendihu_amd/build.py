@@ -28,19 +28,21 @@ ARCH = os.environ.get("FMS_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off: every + - * / rounds as written, as in the reference's
 # g++ -std=c++17 build of its generated code (bit-exact parity).
 # -packed-fp32-ops: no packed FP32 instructions are generated.
-# -amdgpu-sched-strategy=max-ilp: the machine scheduler that maximises
-# instruction-level parallelism for one wave per SIMD. The kernels of large
-# models run at one wave per SIMD anyway (400+ VGPR+AGPR per lane), and with
-# the default occupancy-driven scheduler the fused kernel of the 56-state model
-# computed wrong and run-to-run different states on gfx950 (hipcc of ROCm 7.2),
-# while the same source scheduled for ILP is bit-identical to the oracle
-# (tests/test_gpu_models.py) and faster.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-mllvm", "-amdgpu-sched-strategy=max-ilp",
 ]
+# Large models (more than 8 states): the machine scheduler that maximises
+# instruction-level parallelism for one wave per SIMD. Their kernels run at one
+# wave per SIMD anyway (400+ VGPR+AGPR per lane), and with the default
+# occupancy-driven scheduler the fused kernel of the 56-state model computed
+# wrong and run-to-run different states on gfx950 (hipcc of ROCm 7.2), while
+# the same source scheduled for ILP is bit-identical to the oracle
+# (tests/test_gpu_models.py) and faster. Small models keep the default
+# scheduler, which is faster for them (occupancy matters there).
+LARGE_MODEL_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+LARGE_MODEL_STATES = 8
 
 HH_LIBRARY = "libfms_hodgkin_huxley_1952.so"
 
@@ -57,11 +59,12 @@ def _run(cmd: list[str]) -> None:
 
 
 def compile_library(out_name: str, model_header: str | None = None, model_type: str | None = None,
-                    extra_include: str | None = None) -> str:
+                    extra_include: str | None = None, n_states: int = 4) -> str:
     """Compile fms_lib.hip for one model into ``LIB_DIR/out_name``."""
     os.makedirs(LIB_DIR, exist_ok=True)
     out = os.path.join(LIB_DIR, out_name)
-    cmd = [HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}"]
+    flags = HIP_FLAGS + (LARGE_MODEL_FLAGS if n_states > LARGE_MODEL_STATES else [])
+    cmd = [HIPCC, *flags, f"-I{INCLUDE}", f"-I{CSRC}"]
     if extra_include:
         cmd.append(f"-I{extra_include}")
     if model_header:
@@ -94,7 +97,10 @@ def build_model_library(model_c_file: str, parameters_used_as_algebraic=(), para
                               struct_name=type_name.split("::")[1])
     with open(os.path.join(GEN_DIR, header_name), "w") as f:
         f.write(code)
-    return compile_library(lib_name or f"libfms_{base}.so", header_name, type_name, extra_include=GEN_DIR)
+    from .cellml import parse_opencor_c
+    n_states = parse_opencor_c(text).n_states
+    return compile_library(lib_name or f"libfms_{base}.so", header_name, type_name, extra_include=GEN_DIR,
+                           n_states=n_states)
 
 
 # CellML models shipped with the reference's tests (testing/unit_testing/input),

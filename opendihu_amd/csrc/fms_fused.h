@@ -75,10 +75,16 @@ __device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, doub
 // One reaction half of one node with its gates streamed from HBM: load the
 // gates, n0D Heun steps, store the gates (if `store`), return Vm. The caller
 // runs it on every lane of the wave (lanes past the fibre's end pass
-// store = false), so the large models' reaction code never executes under a
-// partial execution mask.
+// store = false). Not inlined: inlined into the fused kernel (beside its LDS
+// solve and barriers) the 56-state model's reaction computed wrong, in some
+// builds run-to-run different, states on gfx950 with hipcc of ROCm 7.2 under
+// both the default and the ILP scheduler, while the same code as a called
+// function, or in a kernel of its own (k_reaction, k_reaction_fibre), gives
+// the oracle's bits. The call costs scratch spills (the callee has 256 VGPRs
+// and no AGPRs), so for large models the split kernels are the faster exact
+// schedule and the autotuner picks them.
 template <class M, class G, class E>
-__device__ __forceinline__ double streamed_half(G *__restrict__ gates, long long N, long long node, bool store, double V,
+__device__ __noinline__ double streamed_half(G *__restrict__ gates, long long N, long long node, bool store, double V,
                                              const double *__restrict__ fparams, int stim_mode, int stim_param,
                                              double amplitude, bool inset, const unsigned char *flags, double t0,
                                              double dt, int n0D, double vstim) {

@@ -272,6 +272,53 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
 }
 
 // ----------------------------------------------------------------------------
+// reaction with one workgroup per fibre (large models): the fibre's nodes in
+// passes of T lanes (lanes past the end shadow the last node and store
+// nothing), the fibre's flags and parameters uniform over the workgroup
+// ----------------------------------------------------------------------------
+template <class M, class G, class E, int T>
+__global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, double *__restrict__ vm,
+                                                      G *__restrict__ gates, const unsigned char *__restrict__ flags,
+                                                      StepRef sr, int half, int n0D, double vstim,
+                                                      const int *__restrict__ order) {
+  const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
+  const long long N = fb.n_total;
+  const int off = fb.fiber_off[f];
+  const int n = fb.fiber_off[f + 1] - off;
+  const double *row = sr.row();
+  const double t0 = half_start(row, half), dt = row[2];
+  const int sb = fb.stim_begin[f], se = fb.stim_end[f];
+  const double amp = st.mode == 2 ? st.amplitude[f] : 0.0;
+  for (int base = 0; base < n; base += T) {
+    const bool valid = base + (int)threadIdx.x < n;
+    const int li = valid ? base + (int)threadIdx.x : n - 1;
+    const long long i = off + li;
+    const bool inset = li >= sb && li < se;
+    double p[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+    for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
+    const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
+    double V = vm[i];
+    G g[M::NS - 1];
+#pragma unroll
+    for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
+    for (int j = 0; j < n0D; j++) {
+      const int fl = __builtin_amdgcn_readfirstlane((int)flags[(long long)f * n0D + j]);
+      if (!(fl & FLAG_COMPUTE)) continue;   // wave-uniform: one fibre per workgroup
+      const int on = fl & FLAG_STIM;
+      const bool clamp = st.mode == 1 && on && inset;
+      if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? amp : pbase;
+      heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
+    }
+    if (valid) {
+      vm[i] = V;
+#pragma unroll
+      for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + i] = g[k];
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Row i of the split-step diffusion system for one fibre, exactly as
 // fast_monodomain_solver_compute.tpp:336-429 assembles it (consistent P1 mass):
 //   IE: (K - M/dt) x = -(M/dt) u        CN: (K/2 - M/dt) x = (-K/2 - M/dt) u

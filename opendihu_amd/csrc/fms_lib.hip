@@ -115,6 +115,14 @@ struct FmsSolver {
   int tune_resident = -1;   // -1 auto, 0 stream gates from HBM, 1 keep gates in registers
   int tune_threads = 256;
   int tune_graph = 0;
+  int tune_reaction_fibre = 1;   // split kernels, large models: one workgroup per fibre for the reaction
+
+  // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): HIP events
+  // around every launch on the solver's stream, waited for one by one
+  int prof = 0;
+  hipEvent_t prof_ev[2] = {};
+  double prof_ms[4] = {};
+  long long prof_n[4] = {};
 
   // step cursor of the split kernels and the captured graph of tune_graph split steps
   long long *d_cursor = nullptr;
@@ -207,6 +215,8 @@ void free_all(FmsSolver *s) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : s->adj_ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->prof_ev)
+    if (e) (void)hipEventDestroy(e);
   if (s->stream2) (void)hipStreamDestroy(s->stream2);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -216,6 +226,24 @@ void free_all(FmsSolver *s) {
 // ---------------------------------------------------------------------------
 
 fms::StepRef step_ref(const FmsSolver *s, int q) { return fms::StepRef{s->d_cursor, q}; }
+
+// duration of one launch of a kernel class when per-kernel timing is on
+struct KernelTimer {
+  FmsSolver *s;
+  int kind;
+  KernelTimer(FmsSolver *s_, int kind_) : s(s_), kind(kind_) {
+    if (s->prof) (void)hipEventRecord(s->prof_ev[0], s->stream);
+  }
+  ~KernelTimer() {
+    if (!s->prof) return;
+    float ms = 0;
+    if (hipEventRecord(s->prof_ev[1], s->stream) == hipSuccess && hipEventSynchronize(s->prof_ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, s->prof_ev[0], s->prof_ev[1]) == hipSuccess) {
+      s->prof_ms[kind] += ms;
+      s->prof_n[kind]++;
+    }
+  }
+};
 
 // device step table: the K records of this call plus a copy of the last one
 // (the fused kernel reads one record ahead); the cursor starts at row 0 with
@@ -239,9 +267,14 @@ int upload_steps(FmsSolver *s, const double *steps, int K) {
 
 template <class G, class E>
 int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
+  KernelTimer timer(s, FMS_KERNEL_REACTION);
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
-  if (store_alg)
+  if (Model::NS > 8 && !store_alg && s->tune_reaction_fibre) {
+    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256>), dim3(s->F), dim3(256), 0, s->stream, s->fibers(),
+                       s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D, s->vstim,
+                       s->d_order);
+  } else if (store_alg)
     hipLaunchKernelGGL((fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
                        s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
                        s->vstim, s->d_alg, s->d_alg_idx, s->n_alg_idx);
@@ -255,6 +288,7 @@ int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
 }
 
 int launch_flags(FmsSolver *s, int q, int half, int n0D) {
+  KernelTimer timer(s, FMS_KERNEL_OTHER);
   const int threads = 64;
   hipLaunchKernelGGL(fms::k_stim_flags, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
                      s->stim(), s->F, s->d_flags, step_ref(s, q), half, n0D);
@@ -263,6 +297,7 @@ int launch_flags(FmsSolver *s, int q, int half, int n0D) {
 }
 
 int launch_thomas(FmsSolver *s, int q) {
+  KernelTimer timer(s, FMS_KERNEL_DIFFUSION);
   const int ie = s->scheme == FMS_IMPLICIT_EULER;
   if (s->thomas == FMS_THOMAS_SERIAL) {
     const int threads = 64;
@@ -279,6 +314,7 @@ int launch_thomas(FmsSolver *s, int q) {
 }
 
 int launch_spikes(FmsSolver *s, int q) {
+  KernelTimer timer(s, FMS_KERNEL_OTHER);
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
   hipLaunchKernelGGL(fms::k_spikes, dim3((unsigned)blocks), dim3(threads), 0, s->stream, s->N, s->d_vm,
@@ -336,6 +372,8 @@ template <class G, class E>
 int capture_split_graph(FmsSolver *s, int gsteps, int n0D) {
   drop_graph(s);
   const long long nr = s->n_launch_reaction, nt = s->n_launch_thomas;
+  const int prof = s->prof;
+  s->prof = 0;   // no timing events inside a captured graph
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   for (int q = 0; q < gsteps && !rc; q++) rc = enqueue_split_step<G, E>(s, q, n0D, false);
@@ -345,6 +383,7 @@ int capture_split_graph(FmsSolver *s, int gsteps, int n0D) {
   }
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  s->prof = prof;
   s->n_launch_reaction = nr;
   s->n_launch_thomas = nt;
   if (rc) {
@@ -391,6 +430,7 @@ int run_split(FmsSolver *s, const double *h_steps, int K, int n0D) {
 
 template <class G, class E, int T, int NPT, int MMAX>
 int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr) {
+  KernelTimer timer(s, FMS_KERNEL_FUSED);
   fms::DevSpikes spk;
   spk.enabled = s->spikes;
   spk.threshold = s->spike_th;
@@ -959,6 +999,16 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       if (value < 0 || value > 4096) return fail(FMS_ERR_ARGUMENT, "graph steps must be 0..4096");
       s->tune_graph = value;
       return 0;
+    case FMS_TUNE_REACTION_PER_FIBRE:
+      if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "reaction per fibre is 0 or 1");
+      s->tune_reaction_fibre = value;
+      return 0;
+    case FMS_TUNE_PROFILE_KERNELS:
+      if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "kernel profiling is 0 or 1");
+      for (hipEvent_t &e : s->prof_ev)
+        if (value && !e) HIP_TRY(hipEventCreate(&e));
+      s->prof = value;
+      return 0;
     default:
       return fail(FMS_ERR_ARGUMENT, "unknown tuning key");
   }
@@ -978,6 +1028,19 @@ int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
   float f = 0.f;
   HIP_TRY(hipEventElapsedTime(&f, s->events[from], s->events[to]));
   *ms = f;
+  return 0;
+}
+
+int fms_kernel_times(FmsSolver *s, double *ms, long long *launches, int reset) {
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  for (int k = 0; k < 4; k++) {
+    if (ms) ms[k] = s->prof_ms[k];
+    if (launches) launches[k] = s->prof_n[k];
+    if (reset) {
+      s->prof_ms[k] = 0;
+      s->prof_n[k] = 0;
+    }
+  }
   return 0;
 }
 
@@ -1323,7 +1386,7 @@ void snapshot_free(Snapshot &sn) {
 }
 
 struct Schedule {
-  int kernels, chunk, resident, threads, graph;
+  int kernels, chunk, resident, threads, graph, per_fibre;
 };
 
 int apply_schedule(FmsSolver *s, const Schedule &c) {
@@ -1332,6 +1395,7 @@ int apply_schedule(FmsSolver *s, const Schedule &c) {
   s->tune_resident = c.resident;
   s->tune_threads = c.threads;
   s->tune_graph = c.graph;
+  s->tune_reaction_fibre = c.per_fibre;
   if (c.kernels == FMS_KERNELS_SPLIT && s->thomas == FMS_THOMAS_SERIAL && !s->d_cA) {
     if (dalloc(&s->d_cA, (size_t)s->N) || dalloc(&s->d_dA, (size_t)s->N)) return FMS_ERR_HIP;
   }
@@ -1344,24 +1408,31 @@ std::vector<Schedule> candidates(const FmsSolver *s, int allow_split) {
   std::vector<Schedule> v;
   const bool small = Model::NS <= 8 && s->nmax <= 2048;
   const bool fused_ok = !(s->nmax > 2048 && s->thomas == FMS_THOMAS_SERIAL) && s->nmax <= 4096;
+  const int pf = s->tune_reaction_fibre;
   if (fused_ok) {
-    for (int chunk : {64, 512}) {
-      if (small) {
-        v.push_back({FMS_KERNELS_FUSED, chunk, 1, 256, 0});
-        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0});
+    if (small) {
+      for (int chunk : {64, 512}) {
+        v.push_back({FMS_KERNELS_FUSED, chunk, 1, 256, 0, pf});
+        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0, pf});
         // the partitioned solve's elimination order depends on the threads per
         // fibre, so 128-thread schedules are bit-identical only with the serial solve
         if (s->thomas == FMS_THOMAS_SERIAL && s->nmax <= 1024) {
-          v.push_back({FMS_KERNELS_FUSED, chunk, 1, 128, 0});
-          v.push_back({FMS_KERNELS_FUSED, chunk, 0, 128, 0});
+          v.push_back({FMS_KERNELS_FUSED, chunk, 1, 128, 0, pf});
+          v.push_back({FMS_KERNELS_FUSED, chunk, 0, 128, 0, pf});
         }
-      } else {
-        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0});
       }
+    } else {
+      v.push_back({FMS_KERNELS_FUSED, 512, 0, 256, 0, pf});
     }
   }
-  if (allow_split || !fused_ok)
-    for (int g : {0, 8, 32}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g});
+  if (allow_split || !fused_ok) {
+    if (Model::NS > 8) {
+      for (int fibre : {1, 0})
+        for (int g : {0, 8}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, fibre});
+    } else {
+      for (int g : {0, 8, 32}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, pf});
+    }
+  }
   return v;
 }
 
@@ -1375,7 +1446,8 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
   if (s->d_alg) return fail(FMS_ERR_STATE, "autotune before requesting algebraics for transfer");
   const std::vector<Schedule> cand = candidates(s, allow_split);
   if (cand.empty()) return fail(FMS_ERR_UNSUPPORTED, "no schedule fits this batch");
-  const Schedule keep{s->kernels, s->tune_chunk, s->tune_resident, s->tune_threads, s->tune_graph};
+  const Schedule keep{s->kernels, s->tune_chunk, s->tune_resident, s->tune_threads, s->tune_graph,
+                      s->tune_reaction_fibre};
   Snapshot sn;
   int rc = snapshot_take(s, sn);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1421,6 +1493,7 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
     result->resident_gates = cand[best].resident;
     result->threads = cand[best].threads;
     result->graph_steps = cand[best].graph;
+    result->reaction_per_fibre = cand[best].per_fibre;
     result->ms_per_step = best_ms / n_steps;
     result->n_candidates = (int)cand.size();
   }

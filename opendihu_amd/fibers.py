@@ -157,14 +157,30 @@ class FiberBatch:
                   "fms_set_tuning")
 
     def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None,
-                   threads: int | None = None, graph_steps: int | None = None) -> None:
+                   threads: int | None = None, graph_steps: int | None = None,
+                   reaction_per_fibre: int | None = None) -> None:
         """Schedule knobs. Fused kernel: split steps per launch; gates in registers (1),
         streamed (0) or auto (-1); threads per fibre workgroup (128 or 256). Split
-        kernels: steps per captured hipGraph (0 launches eagerly)."""
+        kernels: steps per captured hipGraph (0 launches eagerly); for large models
+        the reaction with one workgroup per fibre (1) or one lane per node (0)."""
         for key, value in ((L.FMS_TUNE_STEPS_PER_LAUNCH, steps_per_launch), (L.FMS_TUNE_RESIDENT_GATES, resident_gates),
-                           (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps)):
+                           (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps),
+                           (L.FMS_TUNE_REACTION_PER_FIBRE, reaction_per_fibre)):
             if value is not None:
                 self._chk(self.lib.fms_set_tuning(self.handle, key, int(value)), "fms_set_tuning")
+
+    def profile_kernels(self, on: bool = True) -> None:
+        """time every kernel launch with HIP events (for measurement runs)"""
+        self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_PROFILE_KERNELS, int(bool(on))), "fms_set_tuning")
+
+    def kernel_times(self, reset: bool = False) -> dict:
+        """{class: (total ms, launches)} for reaction, diffusion, fused, other"""
+        ms = np.zeros(4)
+        n = np.zeros(4, np.int64)
+        self._chk(self.lib.fms_kernel_times(self.handle, L.as_ptr(ms, L._dp),
+                                            n.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), int(reset)),
+                  "fms_kernel_times")
+        return {k: (float(ms[j]), int(n[j])) for j, k in enumerate(L.FMS_KERNEL_CLASSES)}
 
     def graph_counts(self) -> dict:
         c, r = ctypes.c_longlong(), ctypes.c_longlong()
@@ -183,6 +199,7 @@ class FiberBatch:
         self.config["kernels"] = "fused" if res.kernels == L.FMS_KERNELS_FUSED else "split"
         return dict(kernels=self.config["kernels"], steps_per_launch=res.steps_per_launch,
                     resident_gates=res.resident_gates, threads=res.threads, graph_steps=res.graph_steps,
+                    reaction_per_fibre=res.reaction_per_fibre,
                     ms_per_step=res.ms_per_step, candidates_ms=ms[:res.n_candidates].tolist())
 
     def record_event(self, slot: int) -> None:
