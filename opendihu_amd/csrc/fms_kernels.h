@@ -77,24 +77,27 @@ struct DevStim {
 template <class M, class G, class E, bool ALG = false>
 FMS_HD void heun_node(double &V, G *g, const double *p, double t, double dt, bool clamp, double vstim,
                       double *alg_out = nullptr) {
-  double y[M::NS], r[M::NS], ys[M::NS], rs[M::NS], a[M::NA], as[M::NA];
-  y[0] = V;
+  // the rates are evaluated in FD (FP64, divisions via div_normal)
+  FD y[M::NS], r[M::NS], ys[M::NS], rs[M::NS], a[M::NA], as[M::NA], pp[M::NP > 0 ? M::NP : 1];
 #pragma unroll
-  for (int k = 1; k < M::NS; k++) y[k] = (double)g[k - 1];
-  M::template rhs<double, E>(y, p, t, r, a);
+  for (int k = 0; k < M::NP; k++) pp[k] = FD(p[k]);
+  y[0] = FD(V);
 #pragma unroll
-  for (int k = 0; k < M::NS; k++) ys[k] = y[k] + dt * r[k];
-  if (clamp) ys[0] = vstim;
-  M::template rhs<double, E>(ys, p, t, rs, as);
+  for (int k = 1; k < M::NS; k++) y[k] = FD((double)g[k - 1]);
+  M::template rhs<FD, E>(y, pp, t, r, a);
 #pragma unroll
-  for (int k = 0; k < M::NS; k++) y[k] += 0.5 * dt * (r[k] + rs[k]);
-  if (clamp) y[0] = vstim;
-  V = y[0];
+  for (int k = 0; k < M::NS; k++) ys[k] = FD(y[k].v + dt * r[k].v);
+  if (clamp) ys[0] = FD(vstim);
+  M::template rhs<FD, E>(ys, pp, t, rs, as);
 #pragma unroll
-  for (int k = 1; k < M::NS; k++) g[k - 1] = (G)y[k];
+  for (int k = 0; k < M::NS; k++) y[k] = FD(y[k].v + 0.5 * dt * (r[k].v + rs[k].v));
+  if (clamp) y[0] = FD(vstim);
+  V = y[0].v;
+#pragma unroll
+  for (int k = 1; k < M::NS; k++) g[k - 1] = (G)y[k].v;
   if constexpr (ALG) {
 #pragma unroll
-    for (int k = 0; k < M::NA; k++) alg_out[k] = as[k];
+    for (int k = 0; k < M::NA; k++) alg_out[k] = as[k].v;
   }
 }
 
@@ -534,17 +537,17 @@ __global__ __launch_bounds__(256) void k_eval_rhs(const double *__restrict__ ys,
                                                   double t, int n, double *__restrict__ rs, double *__restrict__ as) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double y[M::NS], r[M::NS], a[M::NA], p[M::NP > 0 ? M::NP : 1];
+  FD y[M::NS], r[M::NS], a[M::NA], p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
-  for (int k = 0; k < M::NS; k++) y[k] = ys[(long long)k * n + i];
+  for (int k = 0; k < M::NS; k++) y[k] = FD(ys[(long long)k * n + i]);
 #pragma unroll
-  for (int k = 0; k < M::NP; k++) p[k] = ps[(long long)k * n + i];
-  M::template rhs<double, E>(y, p, t, r, a);
+  for (int k = 0; k < M::NP; k++) p[k] = FD(ps[(long long)k * n + i]);
+  M::template rhs<FD, E>(y, p, t, r, a);
 #pragma unroll
-  for (int k = 0; k < M::NS; k++) rs[(long long)k * n + i] = r[k];
+  for (int k = 0; k < M::NS; k++) rs[(long long)k * n + i] = r[k].v;
   if (as)
 #pragma unroll
-    for (int k = 0; k < M::NA; k++) as[(long long)k * n + i] = a[k];
+    for (int k = 0; k < M::NA; k++) as[(long long)k * n + i] = a[k].v;
 }
 
 // n_steps Heun steps at n independent points (SoA in place)

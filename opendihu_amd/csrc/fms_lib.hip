@@ -458,9 +458,10 @@ int launch_fused(FmsSolver *s, const double *d_steps, int K, int n0D, const fms:
   const bool resident = s->tune_resident != 0;
   if constexpr (Model::NS <= 8) {
     if (s->tune_threads == 128) {
+      // resident gates for at most 8 nodes per thread: 16 per thread spill to
+      // scratch, and that build of the fused kernel computed wrong states
       const int npt = (s->nmax + 127) / 128;
       if (resident && npt <= 8) return launch_fused_npt<G, E, 128, 8, 16>(s, d_steps, K, n0D, tr);
-      if (resident && npt <= 16) return launch_fused_npt<G, E, 128, 16, 16>(s, d_steps, K, n0D, tr);
       return launch_fused_npt<G, E, 128, 0, 16>(s, d_steps, K, n0D, tr);
     }
   }
@@ -981,14 +982,16 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       return 0;
     case FMS_TUNE_RESIDENT_GATES:
       if (value < -1 || value > 1) return fail(FMS_ERR_ARGUMENT, "resident must be -1, 0 or 1");
-      if (value == 1 && (s->nmax > 2048 || Model::NS > 8))
-        return fail(FMS_ERR_UNSUPPORTED, "resident gates need fibres of <= 2048 nodes and <= 8 states");
+      if (value == 1 && (s->nmax > 8 * s->tune_threads || Model::NS > 8))
+        return fail(FMS_ERR_UNSUPPORTED, "resident gates need <= 8 nodes per thread and <= 8 states");
       s->tune_resident = value;
       return 0;
     case FMS_TUNE_THREADS:
       if (value != 128 && value != 256) return fail(FMS_ERR_ARGUMENT, "threads per fibre must be 128 or 256");
       if (value == 128) {
         if (Model::NS > 8) return fail(FMS_ERR_UNSUPPORTED, "128 threads per fibre are built for models of <= 8 states");
+        if (s->tune_resident == 1 && s->nmax > 1024)
+          return fail(FMS_ERR_UNSUPPORTED, "resident gates with 128 threads per fibre need fibres of <= 1024 nodes");
         if (s->nmax > 2048) return fail(FMS_ERR_UNSUPPORTED, "128 threads per fibre need fibres of <= 2048 nodes");
         if (s->thomas == FMS_THOMAS_SERIAL && s->nmax > 1024)
           return fail(FMS_ERR_UNSUPPORTED, "128 threads per fibre with the serial solve need fibres of <= 1024 nodes");

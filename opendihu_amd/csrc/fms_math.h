@@ -24,9 +24,57 @@
 namespace fms {
 
 // exp/log policy: the reference's default approximation (approximateExponentialFunction=True)
+// IEEE double division without the operand scaling of the general sequence.
+// The device's correctly rounded f64 division is v_div_scale (x2), v_rcp_f64,
+// four FMAs refining the reciprocal, a multiply, the remainder FMA,
+// v_div_fmas and v_div_fixup; v_div_scale leaves its operand unchanged and
+// v_div_fmas is a plain FMA unless the operands are denormal, zero or
+// non-finite, the numerator is below 2^-969 or the exponents differ by 768 or
+// more, and v_div_fixup only handles those cases. For every other operand
+// pair the sequence below performs the same operations on the same values
+// and returns the same, correctly rounded, quotient with three instructions
+// fewer; with a constant divisor the reciprocal refinement folds and the
+// quotient costs a multiply and two FMAs. Model states and rates are far from
+// those ranges (the model tests compare the rates and Heun steps with the
+// host's IEEE division bit for bit).
+FMS_HD double div_normal(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double nb = -b;
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double r1 = __builtin_fma(r0, __builtin_fma(nb, r0, 1.0), r0);
+  const double r2 = __builtin_fma(r1, __builtin_fma(nb, r1, 1.0), r1);
+  const double q = a * r2;
+  return __builtin_fma(__builtin_fma(nb, q, a), r2, q);
+#else
+  return a / b;
+#endif
+}
+
+// FP64 arithmetic type of the model right-hand sides: + - * as double,
+// / through div_normal
+struct FD {
+  double v;
+  FD() = default;
+  FMS_HD constexpr FD(double x) : v(x) {}
+  FMS_HD FD &operator+=(FD o) { v = v + o.v; return *this; }
+  FMS_HD FD &operator-=(FD o) { v = v - o.v; return *this; }
+  FMS_HD FD &operator*=(FD o) { v = v * o.v; return *this; }
+};
+FMS_HD FD operator+(FD a, FD b) { return FD(a.v + b.v); }
+FMS_HD FD operator-(FD a, FD b) { return FD(a.v - b.v); }
+FMS_HD FD operator*(FD a, FD b) { return FD(a.v * b.v); }
+FMS_HD FD operator/(FD a, FD b) { return FD(div_normal(a.v, b.v)); }
+FMS_HD FD operator-(FD a) { return FD(-a.v); }
+FMS_HD bool operator<(FD a, FD b) { return a.v < b.v; }
+FMS_HD bool operator>(FD a, FD b) { return a.v > b.v; }
+FMS_HD bool operator<=(FD a, FD b) { return a.v <= b.v; }
+FMS_HD bool operator>=(FD a, FD b) { return a.v >= b.v; }
+FMS_HD bool operator==(FD a, FD b) { return a.v == b.v; }
+FMS_HD bool operator!=(FD a, FD b) { return a.v != b.v; }
+
 struct ExpApprox {
   template <class T> static FMS_HD T exp(T x) {
-    x = T(1.0) + x / T(1024.);
+    x = T(1.0) + x * T(1. / 1024);   // x/1024 exactly (power of two)
 #pragma unroll
     for (int i = 0; i < 10; i++) x *= x;
     return x;
@@ -61,6 +109,8 @@ struct ExpLibm {
   static FMS_HD float exp(float x) { return ::expf(x); }
   static FMS_HD double log(double x) { return ::log(x); }
   static FMS_HD float log(float x) { return ::logf(x); }
+  static FMS_HD FD exp(FD x) { return FD(::exp(x.v)); }
+  static FMS_HD FD log(FD x) { return FD(::log(x.v)); }
 };
 
 template <class T> FMS_HD T pow2(T x) { return x * x; }

@@ -90,7 +90,7 @@ def test_graph_follows_stimulation_and_spike_changes():
 def test_fused_128_threads_equals_256(thomas, resident):
     """serial solve: bit-identical; partitioned solve: the partition follows the
     thread count (another elimination order), so the bar is the partitioned one"""
-    mesh, stim = uneven(10, lo=300, hi=1000 if thomas == "serial" else 2000, seed=8)
+    mesh, stim = uneven(10, lo=300, hi=1000 if thomas == "serial" or resident else 2000, seed=8)
     runs = []
     for threads in (256, 128):
         g = _batch(mesh, scheme="CN", thomas=thomas, kernels="fused")
