@@ -47,7 +47,7 @@ SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 # HBM bytes per node and launch unit (reaction: one half step; fused: one split
 # step) of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, factors from a 1 GiB copy); the summaries are in profiles/
-MEASURED_TRAFFIC_PER_NODE = {}
+MEASURED_TRAFFIC_PER_NODE = {("c2", "reaction"): 925.9}   # profiles/k_reaction_fibre_shorten56_1024x1481.md
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
