@@ -24,10 +24,10 @@ namespace fms {
 
 template <class E> struct ExpDeriv;
 template <> struct ExpDeriv<ExpApprox> {
-  static FMS_HD double d(double x, double ex) { return ex / (1.0 + x / 1024.); }
+  static FMS_HD FD d(FD x, FD ex) { return ex / (FD(1.0) + x * FD(1.0 / 1024)); }
 };
 template <> struct ExpDeriv<ExpLibm> {
-  static FMS_HD double d(double x, double ex) { (void)x; return ex; }
+  static FMS_HD FD d(FD x, FD ex) { (void)x; return ex; }
 };
 
 // Linearisation of the HH right-hand side at y: the rates r(y), computed with
@@ -54,57 +54,61 @@ struct HHLin {
 
 template <class E>
 FMS_HD HHLin hh_linearize(const double *y, const double *p) {
+  // FD arithmetic (the forward's division), so r(y) has the forward's bits
   HHLin L;
-  const double V = y[0], m = y[1], h = y[2], n = y[3];
-  const double i_stim = p[0], g_na = p[1], g_k = p[2], g_l = p[3];
-  const double constant0 = -75, constant1 = 1;
-  const double constant6 = constant0 + 115.000, constant7 = constant0 - 12.0000, constant8 = constant0 + 10.6130;
+  const FD V = y[0], m = y[1], h = y[2], n = y[3];
+  const FD i_stim = p[0], g_na = p[1], g_k = p[2], g_l = p[3];
+  const FD constant0 = FD(-75), constant1 = FD(1);
+  const FD constant6 = constant0 + FD(115.000), constant7 = constant0 - FD(12.0000);
+  const FD constant8 = constant0 + FD(10.6130);
   // rates, in ModelHH::rhs's operation order
-  const double em = E::exp(-(V + 50.0000) / 10.0000);
-  const double a1 = (-0.100000 * (V + 50.0000)) / (em - 1.00000);
-  const double ebm = E::exp(-(V + 75.0000) / 18.0000);
-  const double a5 = 4.00000 * ebm;
-  L.r[1] = a1 * (1.00000 - m) - a5 * m;
-  const double eah = E::exp(-(V + 75.0000) / 20.0000);
-  const double a2 = 0.0700000 * eah;
-  const double ebh = E::exp(-(V + 45.0000) / 10.0000);
-  const double a6 = 1.00000 / (ebh + 1.00000);
-  L.r[2] = a2 * (1.00000 - h) - a6 * h;
-  const double en = E::exp(-(V + 65.0000) / 10.0000);
-  const double a3 = (-0.0100000 * (V + 65.0000)) / (en - 1.00000);
-  const double ebn = E::exp((V + 75.0000) / 80.0000);
-  const double a7 = 0.125000 * ebn;
-  L.r[3] = a3 * (1.00000 - n) - a7 * n;
-  const double m3 = pow3(m), n4 = pow4(n);
-  const double a0 = g_na * m3 * h * (V - constant6);
-  const double a4 = g_k * n4 * (V - constant7);
-  const double a8 = g_l * (V - constant8);
-  L.r[0] = -(-i_stim + a0 + a4 + a8) / constant1;
+  const FD em = E::exp(-(V + FD(50.0000)) / FD(10.0000));
+  const FD a1 = (FD(-0.100000) * (V + FD(50.0000))) / (em - FD(1.00000));
+  const FD ebm = E::exp(-(V + FD(75.0000)) / FD(18.0000));
+  const FD a5 = FD(4.00000) * ebm;
+  const FD r1 = a1 * (FD(1.00000) - m) - a5 * m;
+  const FD eah = E::exp(-(V + FD(75.0000)) / FD(20.0000));
+  const FD a2 = FD(0.0700000) * eah;
+  const FD ebh = E::exp(-(V + FD(45.0000)) / FD(10.0000));
+  const FD a6 = FD(1.00000) / (ebh + FD(1.00000));
+  const FD r2 = a2 * (FD(1.00000) - h) - a6 * h;
+  const FD en = E::exp(-(V + FD(65.0000)) / FD(10.0000));
+  const FD a3 = (FD(-0.0100000) * (V + FD(65.0000))) / (en - FD(1.00000));
+  const FD ebn = E::exp((V + FD(75.0000)) / FD(80.0000));
+  const FD a7 = FD(0.125000) * ebn;
+  const FD r3 = a3 * (FD(1.00000) - n) - a7 * n;
+  const FD m3 = pow3(m), n4 = pow4(n);
+  const FD a0 = g_na * m3 * h * (V - constant6);
+  const FD a4 = g_k * n4 * (V - constant7);
+  const FD a8 = g_l * (V - constant8);
+  const FD r0 = -(-i_stim + a0 + a4 + a8) / constant1;
+  L.r[0] = r0.v; L.r[1] = r1.v; L.r[2] = r2.v; L.r[3] = r3.v;
   // derivatives (E'(x) of the exponential actually used)
-  const double xm = -(V + 50.0000) / 10.0000, xbm = -(V + 75.0000) / 18.0000, xah = -(V + 75.0000) / 20.0000;
-  const double xbh = -(V + 45.0000) / 10.0000, xn = -(V + 65.0000) / 10.0000, xbn = (V + 75.0000) / 80.0000;
-  const double den_m = em - 1.0, num_m = -0.1 * (V + 50.0);
-  const double dam = (-0.1 * den_m - num_m * (ExpDeriv<E>::d(xm, em) * (-0.1))) / (den_m * den_m);
-  const double dbm = 4.0 * ExpDeriv<E>::d(xbm, ebm) * (-1.0 / 18.0);
-  const double dah = 0.07 * ExpDeriv<E>::d(xah, eah) * (-1.0 / 20.0);
-  const double dbh = -(ExpDeriv<E>::d(xbh, ebh) * (-0.1)) / ((ebh + 1.0) * (ebh + 1.0));
-  const double den_n = en - 1.0, num_n = -0.01 * (V + 65.0);
-  const double dan = (-0.01 * den_n - num_n * (ExpDeriv<E>::d(xn, en) * (-0.1))) / (den_n * den_n);
-  const double dbn = 0.125 * ExpDeriv<E>::d(xbn, ebn) / 80.0;
-  const double m2 = m * m, n3 = n * n * n;
-  L.f0V = -(g_na * m3 * h + g_k * n4 + g_l);
-  L.f0m = -(g_na * 3.0 * m2 * h * (V - constant6));
-  L.f0h = -(g_na * m3 * (V - constant6));
-  L.f0n = -(g_k * 4.0 * n3 * (V - constant7));
-  L.f1V = dam * (1.0 - m) - dbm * m;
-  L.f1m = -a1 - a5;
-  L.f2V = dah * (1.0 - h) - dbh * h;
-  L.f2h = -a2 - a6;
-  L.f3V = dan * (1.0 - n) - dbn * n;
-  L.f3n = -a3 - a7;
-  L.gna = -m3 * h * (V - constant6);
-  L.gk = -n4 * (V - constant7);
-  L.gl = -(V - constant8);
+  const FD xm = -(V + FD(50.0000)) / FD(10.0000), xbm = -(V + FD(75.0000)) / FD(18.0000);
+  const FD xah = -(V + FD(75.0000)) / FD(20.0000), xbh = -(V + FD(45.0000)) / FD(10.0000);
+  const FD xn = -(V + FD(65.0000)) / FD(10.0000), xbn = (V + FD(75.0000)) / FD(80.0000);
+  const FD den_m = em - FD(1.0), num_m = FD(-0.1) * (V + FD(50.0));
+  const FD dam = (FD(-0.1) * den_m - num_m * (ExpDeriv<E>::d(xm, em) * FD(-0.1))) / (den_m * den_m);
+  const FD dbm = FD(4.0) * ExpDeriv<E>::d(xbm, ebm) * FD(-1.0 / 18.0);
+  const FD dah = FD(0.07) * ExpDeriv<E>::d(xah, eah) * FD(-1.0 / 20.0);
+  const FD dbh = -(ExpDeriv<E>::d(xbh, ebh) * FD(-0.1)) / ((ebh + FD(1.0)) * (ebh + FD(1.0)));
+  const FD den_n = en - FD(1.0), num_n = FD(-0.01) * (V + FD(65.0));
+  const FD dan = (FD(-0.01) * den_n - num_n * (ExpDeriv<E>::d(xn, en) * FD(-0.1))) / (den_n * den_n);
+  const FD dbn = FD(0.125) * ExpDeriv<E>::d(xbn, ebn) * FD(1.0 / 80.0);
+  const FD m2 = m * m, n3 = n * n * n;
+  L.f0V = (-(g_na * m3 * h + g_k * n4 + g_l)).v;
+  L.f0m = (-(g_na * FD(3.0) * m2 * h * (V - constant6))).v;
+  L.f0h = (-(g_na * m3 * (V - constant6))).v;
+  L.f0n = (-(g_k * FD(4.0) * n3 * (V - constant7))).v;
+  L.f1V = (dam * (FD(1.0) - m) - dbm * m).v;
+  L.f1m = (-a1 - a5).v;
+  L.f2V = (dah * (FD(1.0) - h) - dbh * h).v;
+  L.f2h = (-a2 - a6).v;
+  L.f3V = (dan * (FD(1.0) - n) - dbn * n).v;
+  L.f3n = (-a3 - a7).v;
+  L.gna = (-m3 * h * (V - constant6)).v;
+  L.gk = (-n4 * (V - constant7)).v;
+  L.gl = (-(V - constant8)).v;
   return L;
 }
 
