@@ -191,10 +191,12 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   const double pf = fb.prefactor[f];
   double *red = sh;          // 16*T: partitioned solve
   double *lw = sh + 16 * T;  // n: lambda_V, then w
+  double *lv = lw + n;       // n: the diffusion's V (IE: V^b, CN: V^a + V^b), for the neighbours
   double p[4];
   for (int k = 0; k < 4; k++) p[k] = fb.params[(long long)f * 4 + k];
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   double l[NPT][4], g[NPT][3];
+  int slot[NPT];
   double pfacc = 0;
   // lanes past the fibre's end shadow its last node: they compute, never store
 #pragma unroll
@@ -204,40 +206,54 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
     for (int k = 0; k < 4; k++) l[q][k] = lam[k * N + off + i];
 #pragma unroll
     for (int k = 0; k < 3; k++) g[q][k] = gacc[k * N + off + i];
+    slot[q] = sensor_slot[off + i];
   }
   for (int s = kc - 1; s >= 0; s--) {
     const double *row = steps + 4 * s;
     const double t = row[0], dts = row[1], dt0 = row[2], dt1 = row[3];
+    // all of the step's tape entries and sensor adjoints are loaded up front
+    // (loading them a step ahead instead measured 1.9x slower: the extra live
+    // registers push the linearisation into AGPRs)
     const double *tp = tape + (long long)s * 9 * N;
-    // sensors, reverse of half 2
+    double yb[NPT][4], y0[NPT][4], va[NPT], dl[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+      const long long node = off + (tid + q * T < n ? tid + q * T : n - 1);
+      yb[q][0] = tp[8 * N + node];
+      yb[q][1] = tp[5 * N + node];
+      yb[q][2] = tp[6 * N + node];
+      yb[q][3] = tp[7 * N + node];
+      y0[q][0] = tp[node];
+      y0[q][1] = tp[N + node];
+      y0[q][2] = tp[2 * N + node];
+      y0[q][3] = tp[3 * N + node];
+      va[q] = tp[4 * N + node];
+      dl[q] = dtrace[((long long)s * fb.n_fibers + f) * n_sensors + (slot[q] >= 0 ? slot[q] : 0)];
+    }
+    // sensors, reverse of half 2 from y^b (V after the diffusion, gates after half 1)
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
       const bool valid = tid + q * T < n;
       const int i = valid ? tid + q * T : n - 1;
-      const long long node = off + i;
       const bool inset = i >= sb && i < se;
-      const int slot = sensor_slot[node];
-      const double dl = dtrace[((long long)s * fb.n_fibers + f) * n_sensors + (slot >= 0 ? slot : 0)];
-      l[q][0] += slot >= 0 ? dl : 0.0;
-      // y^b: V after the diffusion with the gates after half 1
-      const double y[4] = {tp[8 * N + node], tp[5 * N + node], tp[6 * N + node], tp[7 * N + node]};
-      reverse_half<E>(st, f, inset, y, p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
+      l[q][0] += slot[q] >= 0 ? dl[q] : 0.0;
+      if (valid) lv[i] = ie ? yb[q][0] : va[q] + yb[q][0];
+      reverse_half<E>(st, f, inset, yb[q], p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
       if (valid) lw[i] = l[q][0];
     }
     __syncthreads();
     // reverse diffusion
     lds_thomas_part<T, MMAX, true>(lw, n, h, pf, dt1, ie != 0, red, tid);
-    const double *Va = tp + 4 * N + off, *Vb = tp + 8 * N + off;
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
       const bool valid = tid + q * T < n;
       const int i = valid ? tid + q * T : n - 1;
       double a, b, c, d;
       assemble_row(i, n, lw, h, pf, dt1, ie != 0, a, b, c, d);   // d = (B w)_i
-      const double vi = ie ? Vb[i] : Va[i] + Vb[i];
+      const double vi = lv[i];
       double k1v = 0;
-      if (i > 0) k1v += ((ie ? Vb[i - 1] : Va[i - 1] + Vb[i - 1]) - vi) / h[i - 1];
-      if (i < n - 1) k1v += ((ie ? Vb[i + 1] : Va[i + 1] + Vb[i + 1]) - vi) / h[i];
+      if (i > 0) k1v += (lv[i - 1] - vi) / h[i - 1];
+      if (i < n - 1) k1v += (lv[i + 1] - vi) / h[i];
       pfacc += valid ? lw[i] * (ie ? -k1v : -0.5 * k1v) : 0.0;
       l[q][0] = d;
     }
@@ -255,16 +271,15 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
       for (int q = 1; q < T / 64; q++) sum += red[q];
       pf_bar[f] += sum;
     }
-    // (the next step writes red only after the barrier that follows its first
-    // loop, so thread 0's reads above need no barrier of their own)
+    // (every read of lw and lv above precedes that barrier, and the next step
+    // writes red only after the barrier that follows its first loop, so thread
+    // 0's read of red needs no barrier of its own)
     // reverse of half 1
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
       const int i = tid + q * T < n ? tid + q * T : n - 1;
-      const long long node = off + i;
       const bool inset = i >= sb && i < se;
-      const double y0[4] = {tp[node], tp[N + node], tp[2 * N + node], tp[3 * N + node]};
-      reverse_half<E>(st, f, inset, y0, p, t, dt0, n0D, l[q], g[q]);
+      reverse_half<E>(st, f, inset, y0[q], p, t, dt0, n0D, l[q], g[q]);
     }
   }
 #pragma unroll

@@ -1131,7 +1131,7 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     s->spikes = 0;
     const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
-    const size_t lds = (size_t)(16 * T + s->nmax) * sizeof(double);
+    const size_t lds = (size_t)(16 * T + 2 * s->nmax) * sizeof(double);
     hipStream_t main_stream = s->stream;
     for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
