@@ -47,7 +47,10 @@ SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 # HBM bytes per node and launch unit (reaction: one half step; fused: one split
 # step) of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, factors from a 1 GiB copy); the summaries are in profiles/
-MEASURED_TRAFFIC_PER_NODE = {("c2", "reaction"): 925.9}   # profiles/k_reaction_fibre_shorten56_1024x1481.md
+# HBM bytes per node and launch of the dominant kernel, measured with rocprofv3
+# (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB device copy):
+# profiles/k_reaction_fibre_shorten56_1024x1481.md
+MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction"): 925.9}
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
@@ -340,17 +343,22 @@ def main(argv=None):
     g.synchronize()
     times = g.kernel_times()
     g.profile_kernels(False)
-    kclass = max(("reaction", "fused"), key=lambda k: times[k][0])
+    kclass = max(("reaction", "fused", "reaction_pair"), key=lambda k: times[k][0])
     total_ms, n_launch = times[kclass]
     mean_launch_ms = total_ms / max(1, n_launch)
     ops = L.model_ops(g.lib)["ops_rhs"]
-    if kclass == "reaction":
-        # one launch = one reaction half (n0D = 1 Heun step) over all nodes
+    if kclass in ("reaction", "reaction_pair"):
+        # one launch = one reaction half (n0D = 1 Heun step) over all nodes, or
+        # two (the second half of a step and the first of the next) that read
+        # and write the states once; unit = one node-half
+        halves = 2 if kclass == "reaction_pair" else 1
         per_fibre = (schedule or {}).get("reaction_per_fibre", 1)
         kernel = "k_reaction_fibre" if g.n_states > 8 and per_fibre else "k_reaction"
-        units_per_launch = n_total
+        if halves == 2:
+            kernel += " (PAIR: two reaction halves per launch)"
+        units_per_launch = n_total * halves
         f_unit = 2 * ops + 5 * g.n_states
-        b_unit = bytes_per_node_step(g.n_states, "fp64") // 1
+        b_unit = bytes_per_node_step(g.n_states, "fp64") / halves
     else:
         kernel = "k_fused"
         units_per_launch = n_total * 64 / max(1, n_launch)
@@ -358,7 +366,7 @@ def main(argv=None):
         b_unit = bytes_per_node_step(g.n_states, "fp64")
     achieved_tflops = f_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e12
     achieved_gbs = b_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    traffic_per_node = MEASURED_TRAFFIC_PER_NODE.get((args.workload, kclass))
+    traffic_per_node = MEASURED_TRAFFIC_PER_NODE_LAUNCH.get((args.workload, kclass))
     kernel_times = {k: dict(total_ms=v[0], launches=v[1]) for k, v in times.items()}
     g.close()
 
@@ -413,7 +421,7 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": traffic_per_node * units_per_launch if traffic_per_node else None,
+            "traffic": traffic_per_node * n_total if traffic_per_node else None,
             "ops_per_node_and_launch_unit": f_unit,
             "nodes_x_units_per_launch": units_per_launch,
             "mean_launch_ms": mean_launch_ms,

@@ -165,6 +165,9 @@ int fms_last_error(char *buf, int len);
 int fms_set_tuning(FmsSolver *solver, int key, int value);
 
 #define FMS_TUNE_PROFILE_KERNELS 7    /* 1: time every launch with HIP events (serialises the host with the GPU) */
+#define FMS_TUNE_PAIR_HALVES 8        /* split kernels: 1 (default) runs the second reaction half of step q and the
+                                         first of step q+1 (no solve between them) as one launch that reads and
+                                         writes the states once; 0: one launch per half. Same arithmetic. */
 
 /* accumulated launch durations in ms and launch counts per kernel class while
    FMS_TUNE_PROFILE_KERNELS is on (graph replays are not timed); reset != 0
@@ -173,7 +176,10 @@ int fms_set_tuning(FmsSolver *solver, int key, int value);
 #define FMS_KERNEL_DIFFUSION 1
 #define FMS_KERNEL_FUSED 2
 #define FMS_KERNEL_OTHER 3
-int fms_kernel_times(FmsSolver *solver, double ms[4], long long launches[4], int reset);
+#define FMS_KERNEL_REACTION_PAIR 4    /* paired reaction halves (FMS_TUNE_PAIR_HALVES) */
+#define FMS_KERNEL_CLASSES 5
+int fms_kernel_times(FmsSolver *solver, double ms[FMS_KERNEL_CLASSES], long long launches[FMS_KERNEL_CLASSES],
+                     int reset);
 
 /* graphs captured and graph replays issued so far (split kernels) */
 int fms_graph_counts(FmsSolver *solver, long long *captures, long long *replays);

@@ -19,15 +19,6 @@
 
 namespace fms {
 
-struct DevSpikes {
-  int enabled;
-  double threshold;
-  double *vm_prev;      // [N]
-  int *first_step;      // [N]
-  int *count;           // [N]
-  int step0;            // global index of the step before this launch's first step
-};
-
 // sensor traces and the adjoint tape (both optional)
 struct DevTrace {
   const int *sensor_slot;   // [N] sensor number of each node, -1 elsewhere; nullptr: no traces

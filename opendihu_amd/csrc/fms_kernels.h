@@ -10,7 +10,12 @@
 //
 // Kernels here:
 //   k_stim_flags      one thread per fibre: stimulation state machine per Heun step
+//                     (k_stim_flags_pair: for two consecutive reaction halves)
 //   k_reaction        one node per lane, coalesced SoA streaming, n0D Heun steps
+//                     (PAIR: the second half of step q and the first of step q+1,
+//                     which follow each other without a solve between them, in
+//                     one pass over the states)
+//   k_reaction_fibre  the same with one workgroup per fibre (large models)
 //   k_thomas_serial   one lane per fibre, the reference's Thomas recurrence (bit-exact)
 //   k_thomas_part     one workgroup per fibre, partitioned Thomas + PCR in LDS
 //   k_fused           one workgroup per fibre, many split steps; Vm lives in LDS,
@@ -171,6 +176,26 @@ __device__ __forceinline__ unsigned char stim_flag(const DevStim &s, int f, doub
 // replays over successive rows (k_cursor_advance) of whatever table the
 // current call uploaded.
 // ----------------------------------------------------------------------------
+// spike detector state (optional): upward crossings of the threshold by Vm
+// after each split step
+struct DevSpikes {
+  int enabled;
+  double threshold;
+  double *vm_prev;      // [N]
+  int *first_step;      // [N]
+  int *count;           // [N]
+  int step0;            // fused kernel: global index of the step before the launch's first step
+};
+
+__device__ __forceinline__ void spike_update(const DevSpikes &spk, long long i, double v, int step) {
+  const double prev = spk.vm_prev[i];
+  if (prev < spk.threshold && v >= spk.threshold) {
+    spk.count[i]++;
+    if (spk.first_step[i] < 0) spk.first_step[i] = step;
+  }
+  spk.vm_prev[i] = v;
+}
+
 struct StepRef {
   const long long *cursor;
   int q;   // step within the launch sequence
@@ -206,6 +231,19 @@ __global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, Ste
   for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, t0 + j * dt);
 }
 
+// flags of two consecutive reaction halves, the second half of step q and the
+// first half of step q+1, in the state machine's time order: flags[f*n0D + j]
+// and flags[(n_fibers + f)*n0D + j]
+__global__ void k_stim_flags_pair(DevStim st, int n_fibers, unsigned char *flags, StepRef sr, int n0D) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fibers) return;
+  const double *row = sr.row();
+  const double ta = half_start(row, 1), dta = row[2];
+  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, ta + j * dta);
+  const double tb = half_start(row + 4, 0), dtb = row[4 + 2];
+  for (int j = 0; j < n0D; j++) flags[((long long)n_fibers + f) * n0D + j] = stim_flag(st, f, tb + j * dtb);
+}
+
 // ----------------------------------------------------------------------------
 // reaction: one node per lane, n0D Heun steps, SoA coalesced streaming
 //
@@ -218,13 +256,18 @@ __global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, Ste
 // of VALU instructions under register pressure) gave wrong and run-to-run
 // different results on gfx950; this form gives the oracle's bits.
 // ALG: also return the algebraics of the last Heun step (algebraicsForTransfer).
+// PAIR: the two reaction halves between two diffusion solves in one pass, the
+// second half of step q (flags[0 .. F*n0D)) and the first half of step q+1
+// (flags[F*n0D .. 2*F*n0D)), with the spike detector of step q between them:
+// the states are read and written once for both, the arithmetic is unchanged.
 // ----------------------------------------------------------------------------
-template <class M, class G, class E, bool ALG>
+template <class M, class G, class E, bool ALG, bool PAIR = false>
 __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, double *__restrict__ vm,
                                                   G *__restrict__ gates, const unsigned char *__restrict__ flags,
                                                   StepRef sr, int half, int n0D, double vstim,
                                                   double *__restrict__ alg_transfer, const int *__restrict__ alg_idx,
-                                                  int n_alg_transfer) {
+                                                  int n_alg_transfer, DevSpikes spk) {
+  static_assert(!(ALG && PAIR), "algebraics are returned by an unpaired last half");
   const long long N = fb.n_total;
   const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i0 < N;
@@ -232,7 +275,8 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   const int f = fb.node_fiber[i];
   const int li = (int)(i - fb.fiber_off[f]);
   const double *row = sr.row();
-  const double t0 = half_start(row, half), dt = row[2];
+  const double t0 = half_start(row, PAIR ? 1 : half), dt = row[2];
+  const double t1 = PAIR ? half_start(row + 4, 0) : 0.0, dt1 = PAIR ? row[4 + 2] : 0.0;
   double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
   for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
@@ -243,18 +287,26 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
   const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   double alg[ALG ? M::NA : 1] = {};
-  for (int j = 0; j < n0D; j++) {
-    const unsigned char fl = flags[(long long)f * n0D + j];
+  const int nj = PAIR ? 2 * n0D : n0D;
+  for (int j = 0; j < nj; j++) {
+    const bool second = PAIR && j >= n0D;
+    const int jj = second ? j - n0D : j;
+    const unsigned char fl = flags[(second ? (long long)fb.n_fibers * n0D : 0LL) + (long long)f * n0D + jj];
     const bool compute = (fl & FLAG_COMPUTE) != 0;
-    if (__builtin_amdgcn_ballot_w64(compute) == 0) continue;   // wave-uniform
-    const int on = fl & FLAG_STIM;
-    const bool clamp = compute && st.mode == 1 && on && inset;
-    const double dtj = compute ? dt : 0.0;
-    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[f] : pbase;
-    if constexpr (ALG) {
-      heun_node<M, G, E, true>(V, g, p, t0 + j * dt, dtj, clamp, vstim, alg);
-    } else {
-      heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dtj, clamp, vstim);
+    if (__builtin_amdgcn_ballot_w64(compute) != 0) {   // wave-uniform
+      const int on = fl & FLAG_STIM;
+      const bool clamp = compute && st.mode == 1 && on && inset;
+      const double dtj = compute ? (second ? dt1 : dt) : 0.0;
+      const double tj = second ? t1 + jj * dt1 : t0 + jj * dt;
+      if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[f] : pbase;
+      if constexpr (ALG) {
+        heun_node<M, G, E, true>(V, g, p, tj, dtj, clamp, vstim, alg);
+      } else {
+        heun_node<M, G, E, false>(V, g, p, tj, dtj, clamp, vstim);
+      }
+    }
+    if constexpr (PAIR) {
+      if (j == n0D - 1 && spk.enabled && valid) spike_update(spk, i, V, (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1));
     }
   }
   if (valid) {
@@ -277,21 +329,25 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
 // ----------------------------------------------------------------------------
 // reaction with one workgroup per fibre (large models): the fibre's nodes in
 // passes of T lanes (lanes past the end shadow the last node and store
-// nothing), the fibre's flags and parameters uniform over the workgroup
+// nothing), the fibre's flags and parameters uniform over the workgroup.
+// PAIR as for k_reaction.
 // ----------------------------------------------------------------------------
-template <class M, class G, class E, int T>
+template <class M, class G, class E, int T, bool PAIR = false>
 __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, double *__restrict__ vm,
                                                       G *__restrict__ gates, const unsigned char *__restrict__ flags,
                                                       StepRef sr, int half, int n0D, double vstim,
-                                                      const int *__restrict__ order) {
+                                                      const int *__restrict__ order, DevSpikes spk) {
   const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
   const long long N = fb.n_total;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
   const double *row = sr.row();
-  const double t0 = half_start(row, half), dt = row[2];
+  const double t0 = half_start(row, PAIR ? 1 : half), dt = row[2];
+  const double t1 = PAIR ? half_start(row + 4, 0) : 0.0, dt1 = PAIR ? row[4 + 2] : 0.0;
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   const double amp = st.mode == 2 ? st.amplitude[f] : 0.0;
+  const int nj = PAIR ? 2 * n0D : n0D;
+  const int spike_step = PAIR ? (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1) : 0;
   for (int base = 0; base < n; base += T) {
     const bool valid = base + (int)threadIdx.x < n;
     const int li = valid ? base + (int)threadIdx.x : n - 1;
@@ -305,13 +361,20 @@ __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, 
     G g[M::NS - 1];
 #pragma unroll
     for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
-    for (int j = 0; j < n0D; j++) {
-      const int fl = __builtin_amdgcn_readfirstlane((int)flags[(long long)f * n0D + j]);
-      if (!(fl & FLAG_COMPUTE)) continue;   // wave-uniform: one fibre per workgroup
-      const int on = fl & FLAG_STIM;
-      const bool clamp = st.mode == 1 && on && inset;
-      if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? amp : pbase;
-      heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
+    for (int j = 0; j < nj; j++) {
+      const bool second = PAIR && j >= n0D;
+      const int jj = second ? j - n0D : j;
+      const int fl = __builtin_amdgcn_readfirstlane(
+          (int)flags[(second ? (long long)fb.n_fibers * n0D : 0LL) + (long long)f * n0D + jj]);
+      if (fl & FLAG_COMPUTE) {   // wave-uniform: one fibre per workgroup
+        const int on = fl & FLAG_STIM;
+        const bool clamp = st.mode == 1 && on && inset;
+        if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? amp : pbase;
+        heun_node<M, G, E, false>(V, g, p, second ? t1 + jj * dt1 : t0 + jj * dt, second ? dt1 : dt, clamp, vstim);
+      }
+      if constexpr (PAIR) {
+        if (j == n0D - 1 && spk.enabled && valid) spike_update(spk, i, V, spike_step);
+      }
     }
     if (valid) {
       vm[i] = V;

@@ -116,13 +116,14 @@ struct FmsSolver {
   int tune_threads = 256;
   int tune_graph = 0;
   int tune_reaction_fibre = 1;   // split kernels, large models: one workgroup per fibre for the reaction
+  int tune_pair = 1;   // split kernels: the two reaction halves between two solves in one launch
 
   // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): HIP events
   // around every launch on the solver's stream, waited for one by one
   int prof = 0;
   hipEvent_t prof_ev[2] = {};
-  double prof_ms[4] = {};
-  long long prof_n[4] = {};
+  double prof_ms[FMS_KERNEL_CLASSES] = {};
+  long long prof_n[FMS_KERNEL_CLASSES] = {};
 
   // step cursor of the split kernels and the captured graph of tune_graph split steps
   long long *d_cursor = nullptr;
@@ -227,6 +228,29 @@ void free_all(FmsSolver *s) {
 
 fms::StepRef step_ref(const FmsSolver *s, int q) { return fms::StepRef{s->d_cursor, q}; }
 
+fms::DevSpikes dev_spikes(const FmsSolver *s) {
+  fms::DevSpikes spk;
+  memset(&spk, 0, sizeof(spk));
+  spk.enabled = s->spikes;
+  spk.threshold = s->spike_th;
+  spk.vm_prev = s->d_vprev;
+  spk.first_step = s->d_sfirst;
+  spk.count = s->d_scount;
+  spk.step0 = (int)s->step_counter;
+  return spk;
+}
+
+// the flags of one reaction half, or of a pair of halves (second buffer half)
+int ensure_flags(FmsSolver *s, int n0D) {
+  if (s->flags_cap < 2 * s->F * n0D) {
+    if (s->d_flags) HIP_TRY(hipFree(s->d_flags));
+    s->d_flags = nullptr;
+    s->flags_cap = 2 * s->F * n0D;
+    if (dalloc(&s->d_flags, (size_t)s->flags_cap)) return FMS_ERR_HIP;
+  }
+  return 0;
+}
+
 // duration of one launch of a kernel class when per-kernel timing is on
 struct KernelTimer {
   FmsSolver *s;
@@ -270,18 +294,41 @@ int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   KernelTimer timer(s, FMS_KERNEL_REACTION);
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
+  const fms::DevSpikes off{};
   if (Model::NS > 8 && !store_alg && s->tune_reaction_fibre) {
     hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256>), dim3(s->F), dim3(256), 0, s->stream, s->fibers(),
                        s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D, s->vstim,
-                       s->d_order);
+                       s->d_order, off);
   } else if (store_alg)
     hipLaunchKernelGGL((fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
                        s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                       s->vstim, s->d_alg, s->d_alg_idx, s->n_alg_idx);
+                       s->vstim, s->d_alg, s->d_alg_idx, s->n_alg_idx, off);
   else
     hipLaunchKernelGGL((fms::k_reaction<Model, G, E, false>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
                        s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                       s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx);
+                       s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx, off);
+  HIP_TRY(hipGetLastError());
+  s->n_launch_reaction++;
+  return 0;
+}
+
+// the second reaction half of step q and the first of step q+1 in one launch,
+// with the spike detector of step q between them
+template <class G, class E>
+int launch_reaction_pair(FmsSolver *s, int q, int n0D) {
+  KernelTimer timer(s, FMS_KERNEL_REACTION_PAIR);
+  const fms::DevSpikes spk = dev_spikes(s);
+  if (Model::NS > 8 && s->tune_reaction_fibre) {
+    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256, true>), dim3(s->F), dim3(256), 0, s->stream,
+                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), 1, n0D,
+                       s->vstim, s->d_order, spk);
+  } else {
+    const int threads = 256;
+    const long long blocks = (s->N + threads - 1) / threads;
+    hipLaunchKernelGGL((fms::k_reaction<Model, G, E, false, true>), dim3((unsigned)blocks), dim3(threads), 0,
+                       s->stream, s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), 1,
+                       n0D, s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx, spk);
+  }
   HIP_TRY(hipGetLastError());
   s->n_launch_reaction++;
   return 0;
@@ -292,6 +339,15 @@ int launch_flags(FmsSolver *s, int q, int half, int n0D) {
   const int threads = 64;
   hipLaunchKernelGGL(fms::k_stim_flags, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
                      s->stim(), s->F, s->d_flags, step_ref(s, q), half, n0D);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_flags_pair(FmsSolver *s, int q, int n0D) {
+  KernelTimer timer(s, FMS_KERNEL_OTHER);
+  const int threads = 64;
+  hipLaunchKernelGGL(fms::k_stim_flags_pair, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
+                     s->stim(), s->F, s->d_flags, step_ref(s, q), n0D);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -337,6 +393,17 @@ int enqueue_split_step(FmsSolver *s, int q, int n0D, bool store_alg) {
   return 0;
 }
 
+// paired form, between the first half of the call's first step and the second
+// half of its last step: the diffusion of step q, then the second half of step
+// q and the first half of step q+1 in one launch
+template <class G, class E>
+int enqueue_pair_body(FmsSolver *s, int q, int n0D) {
+  int rc;
+  if ((rc = launch_thomas(s, q))) return rc;
+  if ((rc = launch_flags_pair(s, q, n0D))) return rc;
+  return launch_reaction_pair<G, E>(s, q, n0D);
+}
+
 // everything a captured graph of split steps has baked into its kernel arguments
 template <class G, class E>
 std::vector<unsigned char> graph_signature(const FmsSolver *s, int gsteps, int n0D) {
@@ -345,13 +412,14 @@ std::vector<unsigned char> graph_signature(const FmsSolver *s, int gsteps, int n
   struct {
     const void *vm, *gates, *flags, *cursor, *cA, *dA, *vprev, *sfirst, *scount;
     double vstim, spike_th;
-    int spikes, n0D, gsteps, thomas, scheme, nmax, g_is_double, e_is_approx;
+    int spikes, n0D, gsteps, thomas, scheme, nmax, g_is_double, e_is_approx, pair, per_fibre;
   } k;
   memset(&k, 0, sizeof(k));
   k.vm = s->d_vm; k.gates = s->d_gates; k.flags = s->d_flags; k.cursor = s->d_cursor;
   k.cA = s->d_cA; k.dA = s->d_dA; k.vprev = s->d_vprev; k.sfirst = s->d_sfirst; k.scount = s->d_scount;
   k.vstim = s->vstim; k.spike_th = s->spike_th; k.spikes = s->spikes; k.n0D = n0D; k.gsteps = gsteps;
   k.thomas = s->thomas; k.scheme = s->scheme; k.nmax = s->nmax;
+  k.pair = s->tune_pair; k.per_fibre = s->tune_reaction_fibre;
   k.g_is_double = std::is_same<G, double>::value;
   k.e_is_approx = std::is_same<E, fms::ExpApprox>::value;
   std::vector<unsigned char> sig(sizeof(fb) + sizeof(st) + sizeof(k));
@@ -367,7 +435,8 @@ void drop_graph(FmsSolver *s) {
   s->graph_sig.clear();
 }
 
-// capture gsteps split steps and the cursor advance as one graph
+// capture gsteps split steps (paired: gsteps pair bodies) and the cursor
+// advance as one graph
 template <class G, class E>
 int capture_split_graph(FmsSolver *s, int gsteps, int n0D) {
   drop_graph(s);
@@ -376,7 +445,8 @@ int capture_split_graph(FmsSolver *s, int gsteps, int n0D) {
   s->prof = 0;   // no timing events inside a captured graph
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int q = 0; q < gsteps && !rc; q++) rc = enqueue_split_step<G, E>(s, q, n0D, false);
+  for (int q = 0; q < gsteps && !rc; q++)
+    rc = s->tune_pair ? enqueue_pair_body<G, E>(s, q, n0D) : enqueue_split_step<G, E>(s, q, n0D, false);
   if (!rc) {
     hipLaunchKernelGGL(fms::k_cursor_advance, dim3(1), dim3(64), 0, s->stream, s->d_cursor, gsteps);
     if (hipGetLastError() != hipSuccess) rc = fail(FMS_ERR_HIP, "cursor kernel launch failed during capture");
@@ -403,13 +473,21 @@ int capture_split_graph(FmsSolver *s, int gsteps, int n0D) {
 }
 
 // split kernels: K steps, whole groups of tune_graph steps replayed as one
-// captured graph, the rest (and a step that returns algebraics) launched eagerly
+// captured graph, the rest (and a step that returns algebraics) launched eagerly.
+// Paired (tune_pair): the first half of step 0, then K-1 bodies (diffusion of
+// step q, second half of q + first half of q+1; graphs of tune_graph bodies),
+// then the diffusion and second half of step K-1 (which returns algebraics).
 template <class G, class E>
 int run_split(FmsSolver *s, const double *h_steps, int K, int n0D) {
   int rc;
   if ((rc = upload_steps(s, h_steps, K))) return rc;
   const int gs = s->tune_graph;
-  const int graph_steps_max = s->d_alg ? K - 1 : K;
+  const bool pair = s->tune_pair != 0;
+  if (pair) {
+    if ((rc = launch_flags(s, 0, 0, n0D))) return rc;
+    if ((rc = launch_reaction<G, E>(s, 0, 0, n0D, false))) return rc;
+  }
+  const int graph_steps_max = pair ? K - 1 : (s->d_alg ? K - 1 : K);
   int done = 0;
   if (gs > 0 && graph_steps_max >= gs) {
     if (s->graph_sig != graph_signature<G, E>(s, gs, n0D) || !s->graph_exec)
@@ -417,13 +495,23 @@ int run_split(FmsSolver *s, const double *h_steps, int K, int n0D) {
     const int nrep = graph_steps_max / gs;
     for (int r = 0; r < nrep; r++) HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
     done = nrep * gs;
-    s->n_launch_reaction += 2LL * done;
+    s->n_launch_reaction += (pair ? 1LL : 2LL) * done;
     s->n_launch_thomas += done;
     s->n_graph_launches += nrep;
   }
   // the cursor now points at row `done`
-  for (int q = 0; done + q < K; q++)
-    if ((rc = enqueue_split_step<G, E>(s, q, n0D, s->d_alg && done + q == K - 1))) return rc;
+  if (pair) {
+    int q = 0;
+    for (; done + q < K - 1; q++)
+      if ((rc = enqueue_pair_body<G, E>(s, q, n0D))) return rc;
+    if ((rc = launch_thomas(s, q))) return rc;
+    if ((rc = launch_flags(s, q, 1, n0D))) return rc;
+    if ((rc = launch_reaction<G, E>(s, q, 1, n0D, s->d_alg != nullptr))) return rc;
+    if (s->spikes && (rc = launch_spikes(s, q))) return rc;
+  } else {
+    for (int q = 0; done + q < K; q++)
+      if ((rc = enqueue_split_step<G, E>(s, q, n0D, s->d_alg && done + q == K - 1))) return rc;
+  }
   s->step_counter += K;
   return 0;
 }
@@ -431,13 +519,7 @@ int run_split(FmsSolver *s, const double *h_steps, int K, int n0D) {
 template <class G, class E, int T, int NPT, int MMAX>
 int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr) {
   KernelTimer timer(s, FMS_KERNEL_FUSED);
-  fms::DevSpikes spk;
-  spk.enabled = s->spikes;
-  spk.threshold = s->spike_th;
-  spk.vm_prev = s->d_vprev;
-  spk.first_step = s->d_sfirst;
-  spk.count = s->d_scount;
-  spk.step0 = (int)s->step_counter;
+  const fms::DevSpikes spk = dev_spikes(s);
   const size_t lds = (size_t)(16 * T + s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
   hipLaunchKernelGGL((fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->stream, s->fibers(),
                      s->stim(), s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim,
@@ -513,11 +595,7 @@ int run(FmsSolver *s, const double *h_steps, int K, int n0D, bool fused_only) {
 int dispatch(FmsSolver *s, const double *h_steps, int K, int n0D) {
   if (K <= 0) return 0;
   if (n0D < 1) return fail(FMS_ERR_ARGUMENT, "n0D must be >= 1");
-  if (s->flags_cap < s->F * n0D) {
-    if (s->d_flags) HIP_TRY(hipFree(s->d_flags));
-    s->flags_cap = s->F * n0D;
-    if (dalloc(&s->d_flags, (size_t)s->flags_cap)) return FMS_ERR_HIP;
-  }
+  if (ensure_flags(s, n0D)) return FMS_ERR_HIP;
   if (s->precision == FMS_FP64) {
     if (s->exp_mode == FMS_EXP_APPROXIMATE) return run<double, fms::ExpApprox>(s, h_steps, K, n0D);
     return run<double, fms::ExpLibm>(s, h_steps, K, n0D);
@@ -1006,6 +1084,10 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "reaction per fibre is 0 or 1");
       s->tune_reaction_fibre = value;
       return 0;
+    case FMS_TUNE_PAIR_HALVES:
+      if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "paired reaction halves is 0 or 1");
+      s->tune_pair = value;
+      return 0;
     case FMS_TUNE_PROFILE_KERNELS:
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "kernel profiling is 0 or 1");
       for (hipEvent_t &e : s->prof_ev)
@@ -1036,7 +1118,7 @@ int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
 
 int fms_kernel_times(FmsSolver *s, double *ms, long long *launches, int reset) {
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < FMS_KERNEL_CLASSES; k++) {
     if (ms) ms[k] = s->prof_ms[k];
     if (launches) launches[k] = s->prof_n[k];
     if (reset) {
@@ -1198,11 +1280,7 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   const int nseg = (K + Kc - 1) / Kc;
   int rc;
   if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)nseg * Model::NS * N))) return rc;
-  if (s->flags_cap < s->F * n0D) {
-    if (s->d_flags) HIP_TRY(hipFree(s->d_flags));
-    s->flags_cap = s->F * n0D;
-    if (dalloc(&s->d_flags, (size_t)s->flags_cap)) return FMS_ERR_HIP;
-  }
+  if (ensure_flags(s, n0D)) return FMS_ERR_HIP;
   if ((rc = upload_steps(s, steps, K))) return rc;
   s->adj_steps.assign(steps, steps + 4 * (size_t)K);
   s->adj_n0D = n0D;

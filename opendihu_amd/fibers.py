@@ -158,14 +158,17 @@ class FiberBatch:
 
     def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None,
                    threads: int | None = None, graph_steps: int | None = None,
-                   reaction_per_fibre: int | None = None) -> None:
+                   reaction_per_fibre: int | None = None, pair_halves: int | None = None) -> None:
         """Schedule knobs. Fused kernel: split steps per launch; gates in registers (1),
         streamed (0) or auto (-1); threads per fibre workgroup (128 or 256). Split
         kernels: steps per captured hipGraph (0 launches eagerly); for large models
-        the reaction with one workgroup per fibre (1) or one lane per node (0)."""
+        the reaction with one workgroup per fibre (1) or one lane per node (0); the
+        two reaction halves between two solves in one launch (1, default) or one
+        launch per half (0)."""
         for key, value in ((L.FMS_TUNE_STEPS_PER_LAUNCH, steps_per_launch), (L.FMS_TUNE_RESIDENT_GATES, resident_gates),
                            (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps),
-                           (L.FMS_TUNE_REACTION_PER_FIBRE, reaction_per_fibre)):
+                           (L.FMS_TUNE_REACTION_PER_FIBRE, reaction_per_fibre),
+                           (L.FMS_TUNE_PAIR_HALVES, pair_halves)):
             if value is not None:
                 self._chk(self.lib.fms_set_tuning(self.handle, key, int(value)), "fms_set_tuning")
 
@@ -174,9 +177,9 @@ class FiberBatch:
         self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_PROFILE_KERNELS, int(bool(on))), "fms_set_tuning")
 
     def kernel_times(self, reset: bool = False) -> dict:
-        """{class: (total ms, launches)} for reaction, diffusion, fused, other"""
-        ms = np.zeros(4)
-        n = np.zeros(4, np.int64)
+        """{class: (total ms, launches)} for reaction, diffusion, fused, other, reaction_pair"""
+        ms = np.zeros(len(L.FMS_KERNEL_CLASSES))
+        n = np.zeros(len(L.FMS_KERNEL_CLASSES), np.int64)
         self._chk(self.lib.fms_kernel_times(self.handle, L.as_ptr(ms, L._dp),
                                             n.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), int(reset)),
                   "fms_kernel_times")

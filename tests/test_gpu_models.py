@@ -51,10 +51,11 @@ def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=
     if not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}.so")):
         pytest.skip("oracle Shorten model not built")
     mesh, stim = c2(n_fibers=F, n_nodes=n, length_cm=(n - 1) * 0.01, dt=dt, end_time=steps * dt, rate_hz=4000.0)
-    per_node = kernels == "split-per-node"
-    kernels = "split" if per_node else kernels
+    per_node = kernels.startswith("split-per-node")
+    unpaired = kernels.endswith("-unpaired")
+    kernels = "split" if kernels.startswith("split") else kernels
     g = FiberBatch(**mesh, scheme="IE", kernels=kernels, thomas=thomas, precision=precision, library=lib)
-    g.set_tuning(reaction_per_fibre=0 if per_node else 1)
+    g.set_tuning(reaction_per_fibre=0 if per_node else 1, pair_halves=0 if unpaired else 1)
     o = OracleBatch(**mesh, scheme="IE", model=SHORTEN, threads=8)
     for b in (g, o):
         b.set_firing_table_stimulation(**stim)
@@ -63,12 +64,14 @@ def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=
     return g, o
 
 
-@pytest.mark.parametrize("kernels", ["split", "split-per-node", "fused"])
+@pytest.mark.parametrize("kernels", ["split", "split-unpaired", "split-per-node", "split-per-node-unpaired", "fused"])
 def test_shorten_bit_exact(kernels):
     g, o = _shorten_pair(kernels, "serial")
     for k in range(g.n_states):
         np.testing.assert_array_equal(g.get_state(k), o.get_state(k), err_msg=f"state {k}")
     assert (o.spikes()[1] > 0).sum() > 0, "stimulus must trigger action potentials"
+    for a, b in zip(g.spikes(), o.spikes()):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_shorten_partitioned():

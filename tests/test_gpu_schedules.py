@@ -32,12 +32,14 @@ def _firing(batch, mesh):
                                        log_capacity=8192)
 
 
+@pytest.mark.parametrize("pair", [1, 0])
 @pytest.mark.parametrize("thomas", ["serial", "partitioned"])
-def test_graph_replay_equals_eager(thomas):
+def test_graph_replay_equals_eager(thomas, pair):
     mesh, _ = uneven(12, lo=200, hi=600, seed=9)
     runs = {}
     for mode in ("eager", "graph"):
         g = _batch(mesh, scheme="IE", kernels="split", thomas=thomas)
+        g.set_tuning(pair_halves=pair)
         if mode == "graph":
             g.set_tuning(graph_steps=8)
         _firing(g, mesh)
@@ -58,7 +60,8 @@ def test_graph_replay_equals_eager(thomas):
     counts = runs["graph"][3]
     print("graph counts", counts)
     assert counts["captures"] == 1, "the captured graph is reused across advance calls"
-    assert counts["replays"] == sum(k // 8 for k in (37, 8, 50, 3, 400, 1002))
+    # paired: the K-1 bodies between the call's first and last half are replayed
+    assert counts["replays"] == sum((k - pair) // 8 for k in (37, 8, 50, 3, 400, 1002))
     assert runs["eager"][3]["replays"] == 0
 
 
@@ -135,3 +138,38 @@ def test_autotune_keeps_state_and_bits():
         np.testing.assert_array_equal(a, b)
     ra, ta = ref.stimulation_log(), tuned.stimulation_log()
     assert sorted(zip(ra[1].tolist(), ra[0].tolist())) == sorted(zip(ta[1].tolist(), ta[0].tolist()))
+
+
+@pytest.mark.parametrize("thomas", ["serial", "partitioned"])
+def test_paired_halves_equal_unpaired(thomas):
+    """the second reaction half of step q and the first of step q+1 in one
+    launch: states, spikes, stimulation log and algebraics bit-identical to one
+    launch per half, over calls of 1..400 steps, with n0D = 1 and 2, eager and
+    replayed as graphs"""
+    mesh, _ = uneven(10, lo=200, hi=700, seed=12)
+    runs = {}
+    for pair in (0, 1):
+        for gs in (0, 4):
+            g = _batch(mesh, scheme="CN", kernels="split", thomas=thomas)
+            g.set_tuning(pair_halves=pair, graph_steps=gs)
+            _firing(g, mesh)
+            g.enable_spike_detection(-20.0)
+            g.set_algebraics_for_transfer([0, 3])
+            t, dt, out = 0.0, 2e-3, []
+            for k, n0D in ((1, 1), (9, 2), (2, 1), (400, 1), (1, 2), (57, 1)):
+                g.advance(t, dt, k, dt0D=dt / 2 / n0D if n0D > 1 else None, n0D=n0D)
+                t += k * dt
+                out.append((g.get_states(), g.get_algebraics()))
+            runs[(pair, gs)] = (out, g.spikes(), g.stimulation_log())
+            g.close()
+    ref = runs[(0, 0)]
+    assert (ref[1][1] > 0).sum() > 0, "the stimulus must trigger action potentials"
+    for key, run in runs.items():
+        for c, ((a, aa), (b, bb)) in enumerate(zip(ref[0], run[0])):
+            np.testing.assert_array_equal(a, b, err_msg=f"{key} call {c}")
+            np.testing.assert_array_equal(aa, bb, err_msg=f"{key} call {c} algebraics")
+        for a, b in zip(ref[1], run[1]):
+            np.testing.assert_array_equal(a, b, err_msg=f"{key} spikes")
+        ea = sorted(zip(ref[2][1].tolist(), ref[2][0].tolist()))
+        ga = sorted(zip(run[2][1].tolist(), run[2][0].tolist()))
+        assert ea == ga and len(ea) > 0, key
