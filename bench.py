@@ -61,7 +61,10 @@ def bytes_per_node_step(n_states: int, precision: str) -> int:
 
 
 def step_table(t0, dt, n):
-    return np.stack([t0 + np.arange(n) * dt, np.full(n, dt), np.full(n, dt), np.full(n, dt)], 1)
+    """n split-step records (startTime, dtSplit, dt0D, dt1D): one Heun step of
+    dt/2 per reaction half, the reference's step-count rule for dt0D = dt
+    (00_time_stepping_scheme.cpp:19-42)"""
+    return np.stack([t0 + np.arange(n) * dt, np.full(n, dt), np.full(n, dt / 2), np.full(n, dt)], 1)
 
 
 def make_workload(name: str, precision: str, kernels: str):
@@ -165,7 +168,7 @@ def bench_c3(cpu=True):
     from opendihu_amd.build import HH_LIBRARY
     dt, n_steps = 1e-3, 10000
     mesh, stim = uneven(4096, seed=3)
-    trial = np.stack([np.arange(64) * dt, np.full(64, dt), np.full(64, dt), np.full(64, dt)], 1)
+    trial = step_table(0.0, dt, 64)
     res = {}
     for precision in ("fp64", "mixed"):
         g = FiberBatch(**mesh, scheme="CN", precision=precision, kernels="fused", thomas="partitioned",
@@ -211,7 +214,7 @@ def bench_c4():
     mesh, stim = uneven(F, lo=n_nodes, hi=n_nodes, seed=41)
     b = FiberBatch(**mesh, scheme="CN", kernels="fused", thomas="partitioned")
     b.set_current_stimulation(**stim)
-    steps = np.stack([np.arange(n_steps) * dt, np.full(n_steps, dt), np.full(n_steps, dt), np.full(n_steps, dt)], 1)
+    steps = step_table(0.0, dt, n_steps)
     sensors = np.tile(np.round(np.linspace(0, n_nodes - 1, 32)).astype(np.int32), (F, 1))
     prob = AdjointProblem(b, steps, sensors, checkpoint_interval=100, am=AM, cm=CM)
     rng = np.random.default_rng(42)

@@ -25,10 +25,67 @@ namespace fms {
 template <class E> struct ExpDeriv;
 template <> struct ExpDeriv<ExpApprox> {
   static FMS_HD FD d(FD x, FD ex) { return ex / (FD(1.0) + x * FD(1.0 / 1024)); }
+  // power-series coefficients of E(x) - 1 = sum_k c_k x^k, c_k = binom(1024, k) / 1024^k
+  static FMS_HD constexpr double c(int k) {
+    double v = 1.0;
+    for (int j = 1; j < k; j++) v = v * (1024.0 - j) / ((j + 1) * 1024.0);
+    return v;
+  }
 };
 template <> struct ExpDeriv<ExpLibm> {
   static FMS_HD FD d(FD x, FD ex) { (void)x; return ex; }
+  static FMS_HD constexpr double c(int k) {   // 1 / k!
+    double v = 1.0;
+    for (int j = 2; j <= k; j++) v /= j;
+    return v;
+  }
 };
+
+// u(x) = x / (E(x) - 1) has a removable singularity at x = 0: the HH rates
+// alpha_m = u(-(V+50)/10) and alpha_n = u(-(V+65)/10) / 10. The quotient rule
+// u' = (E - 1 - x E') / (E - 1)^2 loses its accuracy near 0: the numerator is
+// about -x^2/2, while E - 1 and x E' each carry ~1e-13 absolute rounding error
+// (the approximate exponential raises a rounded 1 + x/1024 to the 1024th
+// power), so the relative error is ~2e-13 / x^2 - O(1) a few hundredths of a
+// millivolt from the singular voltage, a point every action potential's
+// upstroke passes. For |x| < 0.05, u' is taken from its own power series
+// u'(x) = sum_j d_j x^j, degree 7 (truncation ~1e-18 relative at |x| = 0.05;
+// the quotient rule's error there is 1e-10). The coefficients follow at
+// compile time from the series of E: with s(x) = (E(x) - 1)/x = sum_k c_{k+1} x^k,
+// u = 1/s term by term, d_j = (j+1) u_{j+1}.
+template <class E>
+constexpr double du_coeff(int j) {
+  double s[9] = {}, u[9] = {};
+  for (int k = 0; k < 9; k++) s[k] = ExpDeriv<E>::c(k + 1);
+  u[0] = 1.0 / s[0];
+  for (int m = 1; m < 9; m++) {
+    double acc = 0.0;
+    for (int k = 1; k <= m; k++) acc += s[k] * u[m - k];
+    u[m] = -acc / s[0];
+  }
+  return (j + 1) * u[j + 1];
+}
+
+// against the exact rational coefficients (binomial series inverted in fractions)
+static_assert(du_coeff<ExpApprox>(0) == -0.49951171875, "d0 = -binom(1024,2)/1024^2");
+static_assert(du_coeff<ExpApprox>(1) - 0.16666650772094727 < 1e-17 &&
+                  du_coeff<ExpApprox>(1) - 0.16666650772094727 > -1e-17, "d1");
+static_assert(du_coeff<ExpApprox>(3) - -0.0055554495918386237 < 1e-18 &&
+                  du_coeff<ExpApprox>(3) - -0.0055554495918386237 > -1e-18, "d3");
+static_assert(du_coeff<ExpApprox>(7) - -6.6127054049909082e-06 < 1e-20 &&
+                  du_coeff<ExpApprox>(7) - -6.6127054049909082e-06 > -1e-20, "d7");
+static_assert(du_coeff<ExpLibm>(0) == -0.5, "Bernoulli: B1 = -1/2");
+static_assert(du_coeff<ExpLibm>(2) < 1e-16 && du_coeff<ExpLibm>(2) > -1e-16, "B3 = 0 (to the rounding of the inversion)");
+static_assert(du_coeff<ExpLibm>(5) - 1.0 / 5040 < 1e-17 && du_coeff<ExpLibm>(5) - 1.0 / 5040 > -1e-17, "B6 / 5! = 1/5040");
+
+template <class E>
+__device__ __forceinline__ double du_near_zero(double x) {
+  constexpr double d0 = du_coeff<E>(0), d1 = du_coeff<E>(1), d2 = du_coeff<E>(2), d3 = du_coeff<E>(3);
+  constexpr double d4 = du_coeff<E>(4), d5 = du_coeff<E>(5), d6 = du_coeff<E>(6), d7 = du_coeff<E>(7);
+  return d0 + x * (d1 + x * (d2 + x * (d3 + x * (d4 + x * (d5 + x * (d6 + x * d7))))));
+}
+
+__device__ __forceinline__ bool near_removable(double x) { return x < 0.05 && x > -0.05; }
 
 // Linearisation of the HH right-hand side at y: the rates r(y), computed with
 // the operations of ModelHH::rhs in its order (so y* = y + dt r(y) has the
@@ -53,7 +110,7 @@ struct HHLin {
 };
 
 template <class E>
-FMS_HD HHLin hh_linearize(const double *y, const double *p) {
+__device__ __forceinline__ HHLin hh_linearize(const double *y, const double *p) {
   // FD arithmetic (the forward's division), so r(y) has the forward's bits
   HHLin L;
   const FD V = y[0], m = y[1], h = y[2], n = y[3];
@@ -87,13 +144,25 @@ FMS_HD HHLin hh_linearize(const double *y, const double *p) {
   const FD xm = -(V + FD(50.0000)) / FD(10.0000), xbm = -(V + FD(75.0000)) / FD(18.0000);
   const FD xah = -(V + FD(75.0000)) / FD(20.0000), xbh = -(V + FD(45.0000)) / FD(10.0000);
   const FD xn = -(V + FD(65.0000)) / FD(10.0000), xbn = (V + FD(75.0000)) / FD(80.0000);
-  const FD den_m = em - FD(1.0), num_m = FD(-0.1) * (V + FD(50.0));
-  const FD dam = (FD(-0.1) * den_m - num_m * (ExpDeriv<E>::d(xm, em) * FD(-0.1))) / (den_m * den_m);
+  // d alpha_m / dV = -0.1 u'(xm), d alpha_n / dV = -0.01 u'(xn) (u above): the
+  // quotient rule u' = (E - 1 - x E') / (E - 1)^2, and the series within
+  // |x| < 0.05 of a singular voltage
+  const FD den_m = em - FD(1.0);
+  double dum = ((den_m - xm * ExpDeriv<E>::d(xm, em)) / (den_m * den_m)).v;
   const FD dbm = FD(4.0) * ExpDeriv<E>::d(xbm, ebm) * FD(-1.0 / 18.0);
   const FD dah = FD(0.07) * ExpDeriv<E>::d(xah, eah) * FD(-1.0 / 20.0);
   const FD dbh = -(ExpDeriv<E>::d(xbh, ebh) * FD(-0.1)) / ((ebh + FD(1.0)) * (ebh + FD(1.0)));
-  const FD den_n = en - FD(1.0), num_n = FD(-0.01) * (V + FD(65.0));
-  const FD dan = (FD(-0.01) * den_n - num_n * (ExpDeriv<E>::d(xn, en) * FD(-0.1))) / (den_n * den_n);
+  const FD den_n = en - FD(1.0);
+  double dun = ((den_n - xn * ExpDeriv<E>::d(xn, en)) / (den_n * den_n)).v;
+  // a lane is near at most one of the two singular voltages (1.5 apart in x).
+  // The polynomial is evaluated on every lane and selected: a branch here (even
+  // a wave-uniform one) splits the unrolled per-node code into blocks that the
+  // scheduler cannot interleave, which measured 1.9x slower at one wave per SIMD.
+  const bool near_m = near_removable(xm.v), near_n = near_removable(xn.v);
+  const double dus = du_near_zero<E>(near_m ? xm.v : xn.v);
+  dum = near_m ? dus : dum;
+  dun = near_n ? dus : dun;
+  const FD dam = FD(-0.1 * dum), dan = FD(-0.01 * dun);
   const FD dbn = FD(0.125) * ExpDeriv<E>::d(xbn, ebn) * FD(1.0 / 80.0);
   const FD m2 = m * m, n3 = n * n * n;
   L.f0V = (-(g_na * m3 * h + g_k * n4 + g_l)).v;
@@ -115,7 +184,7 @@ FMS_HD HHLin hh_linearize(const double *y, const double *p) {
 // reverse of one Heun step from its input y (no clamp):
 // lam[4] in: adjoint of the output, out: adjoint of y; gacc[3] += parameter gradient
 template <class E>
-FMS_HD void hh_heun_step_reverse(const double *y, const double *p, double dt, double *lam, double *gacc) {
+__device__ __forceinline__ void hh_heun_step_reverse(const double *y, const double *p, double dt, double *lam, double *gacc) {
   const HHLin Ly = hh_linearize<E>(y, p);
   double ys[4];
   for (int k = 0; k < 4; k++) ys[k] = y[k] + dt * Ly.r[k];

@@ -31,6 +31,26 @@ def _Ed(x):
     return _E(x) / (1.0 + x / 1024.0)
 
 
+# coefficients of E(x) - 1 = sum_k C[k] x^k, C[k] = binom(1024, k) / 1024^k
+_C = [0.0, 1.0]
+for _k in range(1, 8):
+    _C.append(_C[-1] * (1024 - _k) / ((_k + 1) * 1024))
+
+
+def _du(x):
+    """d/dx [x / (E(x) - 1)]: the quotient rule, except for |x| < 0.05, where its
+    numerator E - 1 - x E' (about -x^2/2) cancels catastrophically and both it
+    and E - 1 are summed from their power series (eight terms)"""
+    x = np.asarray(x, float)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        den = _E(x) - 1
+        direct = (den - x * _Ed(x)) / den ** 2
+        sden = sum(_C[k] * x ** k for k in range(1, 9))
+        snum = -sum((k - 1) * _C[k] * x ** k for k in range(2, 9))
+        series = np.where(x == 0, -_C[2], snum / sden ** 2)
+    return np.where(np.abs(x) < 0.05, series, direct)
+
+
 def rhs(y, p):
     V, m, h, n = y
     ist, gna, gk, gl = p
@@ -50,7 +70,7 @@ def vjp(y, p, w):
     _, gna, gk, gl = p
     xm = -(V + 50) / 10
     num, den = -0.1 * (V + 50), _E(xm) - 1
-    am, dam = num / den, (-0.1 * den - num * _Ed(xm) * (-0.1)) / den ** 2
+    am, dam = num / den, -0.1 * _du(xm)
     xbm = -(V + 75) / 18
     bm, dbm = 4 * _E(xbm), 4 * _Ed(xbm) * (-1 / 18)
     xah = -(V + 75) / 20
@@ -60,7 +80,7 @@ def vjp(y, p, w):
     bh, dbh = 1 / (ebh + 1), -(_Ed(xbh) * (-0.1)) / (ebh + 1) ** 2
     xn = -(V + 65) / 10
     numn, denn = -0.01 * (V + 65), _E(xn) - 1
-    an, dan = numn / denn, (-0.01 * denn - numn * _Ed(xn) * (-0.1)) / denn ** 2
+    an, dan = numn / denn, -0.01 * _du(xn)
     xbn = (V + 75) / 80
     bn, dbn = 0.125 * _E(xbn), 0.125 * _Ed(xbn) / 80
     out = np.array([

@@ -24,3 +24,21 @@ def test_reference_adjoint_matches_central_differences(ie):
         e[q] = 1e-6 * th0[q]
         fd = (loss(th0 + e) - loss(th0 - e)) / (2 * e[q])
         assert abs(grad[q] - fd) <= 1e-6 * abs(fd), (q, grad[q], fd)
+
+
+def test_removable_singularity_derivative_exact():
+    """d/dx [x / (E(x) - 1)] with E(x) = (1 + x/1024)^1024 (the HH rates alpha_m,
+    alpha_n near V = -50 and -65 mV) against exact rational arithmetic, on both
+    sides of the switch from the power series to the quotient rule at |x| = 0.05"""
+    from fractions import Fraction
+    for xf in (1e-9, -3e-7, 1e-5, 2.5e-3, -0.0123, 0.0499, -0.0499, 0.0501, 0.3, -1.7):
+        x = Fraction(xf)
+        e = (1 + x / 1024) ** 1024
+        ed = (1 + x / 1024) ** 1023
+        exact = (e - 1 - x * ed) / (e - 1) ** 2
+        got = float(R._du(xf))
+        # series: truncation after eight terms (3e-14 at |x| = 0.05); quotient
+        # rule: its cancellation bound ~2.5e-13 / x^2
+        tol = 1e-13 if abs(xf) < 0.05 else 2.5e-13 / xf ** 2
+        assert abs(got - float(exact)) <= tol * abs(float(exact)), (xf, got, float(exact))
+    assert float(R._du(0.0)) == -R._C[2]

@@ -114,7 +114,7 @@ def test_fused_128_threads_equals_256(thomas, resident):
 def test_autotune_keeps_state_and_bits():
     mesh, _ = uneven(16, lo=300, hi=900, seed=5)
     dt = 2e-3
-    trial = np.stack([np.arange(40) * dt + 1.0, np.full(40, dt), np.full(40, dt), np.full(40, dt)], 1)
+    trial = np.stack([np.arange(40) * dt + 1.0, np.full(40, dt), np.full(40, dt / 2), np.full(40, dt)], 1)
     ref = _batch(mesh, scheme="IE", kernels="split", thomas="serial")
     tuned = _batch(mesh, scheme="IE", kernels="split", thomas="serial")
     for b in (ref, tuned):
