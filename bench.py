@@ -49,8 +49,8 @@ SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 # WRITE_SIZE x 1.000, factors from a 1 GiB copy); the summaries are in profiles/
 # HBM bytes per node and launch of the dominant kernel, measured with rocprofv3
 # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB device copy):
-# profiles/k_reaction_fibre_shorten56_1024x1481.md
-MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction"): 925.9}
+# profiles/k_reaction_fibre_pair_shorten56_1024x1481.md
+MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction_pair"): 928.4, ("c2", "reaction"): 925.8}
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
