@@ -258,8 +258,8 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   const int n = fb.fiber_off[f + 1] - off;
   const double *h = fb.elem_len + off - f;
   const double pf = fb.prefactor[f];
-  double *red = sh;          // 16*T: partitioned solve
-  double *lw = sh + 16 * T;  // n: lambda_V, then w
+  double *red = sh;                          // partitioned solve (pcr_lds_doubles)
+  double *lw = sh + pcr_lds_doubles<T>();    // n: lambda_V, then w
   double *lv = lw + n;       // n: the diffusion's V (IE: V^b, CN: V^a + V^b), for the neighbours
   double p[4];
   for (int k = 0; k < 4; k++) p[k] = fb.params[(long long)f * 4 + k];

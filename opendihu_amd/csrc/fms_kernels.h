@@ -533,41 +533,44 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     for (int k = 0; k < MMAX; k++)
       if (k == cnt - 1) { la = aa[k]; lc = cc[k]; ld = dd[k]; }
   }
-  double *A0 = red, *B0 = red + R, *C0 = red + 2 * R, *D0 = red + 3 * R;
-  double *A1 = red + 4 * R, *B1 = red + 5 * R, *C1 = red + 6 * R, *D1 = red + 7 * R;
-  A0[2 * tid] = fa; B0[2 * tid] = 1.0; C0[2 * tid] = fc; D0[2 * tid] = fd;
-  A0[2 * tid + 1] = la; B0[2 * tid + 1] = 1.0; C0[2 * tid + 1] = lc; D0[2 * tid + 1] = ld;
+  // reduced system of the 2P chunk end rows, each row scaled to a unit
+  // diagonal: a x_{i-1} + x_i + c x_{i+1} = d. PCR keeps the rows normalised
+  // (one reciprocal per row and level), so 6 arrays of R doubles suffice
+  double *A0 = red, *C0 = red + R, *D0 = red + 2 * R;
+  double *A1 = red + 3 * R, *C1 = red + 4 * R, *D1 = red + 5 * R;
+  A0[2 * tid] = fa; C0[2 * tid] = fc; D0[2 * tid] = fd;
+  A0[2 * tid + 1] = la; C0[2 * tid + 1] = lc; D0[2 * tid + 1] = ld;
   __syncthreads();
   for (int s = 1; s < R; s <<= 1) {
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int i = tid + q * T;
       const int im = i - s, ip = i + s;
-      double na = 0, nc = 0, nb = B0[i], nd = D0[i];
+      double na = 0, nc = 0, nb = 1.0, nd = D0[i];
       if (im >= 0) {
-        const double al = -A0[i] / B0[im];
+        const double al = -A0[i];
         na = al * A0[im];
         nb += al * C0[im];
         nd += al * D0[im];
       }
       if (ip < R) {
-        const double ga = -C0[i] / B0[ip];
+        const double ga = -C0[i];
         nc = ga * C0[ip];
         nb += ga * A0[ip];
         nd += ga * D0[ip];
       }
-      A1[i] = na; B1[i] = nb; C1[i] = nc; D1[i] = nd;
+      const double r = 1.0 / nb;
+      A1[i] = na * r; C1[i] = nc * r; D1[i] = nd * r;
     }
     __syncthreads();
     double *t;
     t = A0; A0 = A1; A1 = t;
-    t = B0; B0 = B1; B1 = t;
     t = C0; C0 = C1; C1 = t;
     t = D0; D0 = D1; D1 = t;
   }
   if (cnt > 0) {
-    const double xf = D0[2 * tid] / B0[2 * tid];
-    const double xl = D0[2 * tid + 1] / B0[2 * tid + 1];
+    const double xf = D0[2 * tid];
+    const double xl = D0[2 * tid + 1];
     lv[lo] = xf;
     lv[lo + cnt - 1] = xl;
 #pragma unroll
@@ -576,6 +579,9 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
   }
   __syncthreads();
 }
+
+// LDS doubles of the partitioned solve's reduced system (6 arrays of 2T)
+template <int T> constexpr int pcr_lds_doubles() { return 12 * T; }
 
 // one workgroup per fibre: load Vm to LDS, partitioned solve, store
 template <int T, int MMAX>
@@ -587,7 +593,7 @@ __global__ __launch_bounds__(T) void k_thomas_part(DevFibers fb, double *vm, Ste
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
   double *red = sh;
-  double *lv = sh + 8 * 2 * T;
+  double *lv = sh + pcr_lds_doubles<T>();
   for (int i = tid; i < n; i += T) lv[i] = vm[off + i];
   __syncthreads();
   lds_thomas_part<T, MMAX>(lv, n, fb.elem_len + off - f, fb.prefactor[f], dt, ie != 0, red, tid);

@@ -360,7 +360,7 @@ int launch_thomas(FmsSolver *s, int q) {
     hipLaunchKernelGGL(fms::k_thomas_serial, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
                        s->fibers(), s->d_vm, s->d_cA, s->d_dA, step_ref(s, q), ie);
   } else {
-    const size_t lds = (size_t)(16 * PART_T + s->nmax) * sizeof(double);
+    const size_t lds = (size_t)(fms::pcr_lds_doubles<PART_T>() + s->nmax) * sizeof(double);
     hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 16>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
                        s->d_vm, step_ref(s, q), ie);
   }
@@ -520,7 +520,8 @@ template <class G, class E, int T, int NPT, int MMAX>
 int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const fms::DevTrace &tr) {
   KernelTimer timer(s, FMS_KERNEL_FUSED);
   const fms::DevSpikes spk = dev_spikes(s);
-  const size_t lds = (size_t)(16 * T + s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
+  const int red = s->thomas == FMS_THOMAS_SERIAL ? 16 * T : fms::pcr_lds_doubles<T>();
+  const size_t lds = (size_t)(red + s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
   hipLaunchKernelGGL((fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->stream, s->fibers(),
                      s->stim(), s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim,
                      s->scheme == FMS_IMPLICIT_EULER ? 1 : 0, s->thomas == FMS_THOMAS_SERIAL ? 1 : 0, s->d_order,
@@ -1213,7 +1214,7 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     s->spikes = 0;
     const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
-    const size_t lds = (size_t)(16 * T + 2 * s->nmax) * sizeof(double);
+    const size_t lds = (size_t)(fms::pcr_lds_doubles<T>() + 2 * s->nmax) * sizeof(double);
     hipStream_t main_stream = s->stream;
     for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
