@@ -123,14 +123,17 @@ def test_torch_autograd_function():
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
-    prob, theta_true, theta0 = _problem(n_steps=400, interval=400)
+    """bit-identical gradients for one 1300-step segment and for segments of
+    37 and 650 steps"""
+    prob, theta_true, theta0 = _problem(n_steps=1300, interval=1300)
     target = prob.forward(theta_true)
     tr = prob.forward(theta0)
     g1 = prob.backward(tr - target)
-    prob.interval = 37
-    tr = prob.forward(theta0)
-    g2 = prob.backward(tr - target)
-    np.testing.assert_array_equal(g1, g2)
+    for interval in (37, 650):
+        prob.interval = interval
+        tr = prob.forward(theta0)
+        g2 = prob.backward(tr - target)
+        np.testing.assert_array_equal(g1, g2, err_msg=f"interval {interval}")
 
 
 @pytest.mark.parametrize("scheme", ["CN", "IE"])
