@@ -168,6 +168,9 @@ int fms_set_tuning(FmsSolver *solver, int key, int value);
 #define FMS_TUNE_PAIR_HALVES 8        /* split kernels: 1 (default) runs the second reaction half of step q and the
                                          first of step q+1 (no solve between them) as one launch that reads and
                                          writes the states once; 0: one launch per half. Same arithmetic. */
+#define FMS_TUNE_REACTION_THREADS 9   /* split kernels, models of > 8 states, reaction per fibre: threads per fibre
+                                         workgroup, 256 (default) or 64 (one wave: four fibres share a CU and their
+                                         loads do not coincide). Same arithmetic. */
 
 /* accumulated launch durations in ms and launch counts per kernel class while
    FMS_TUNE_PROFILE_KERNELS is on (graph replays are not timed); reset != 0
@@ -189,7 +192,8 @@ int fms_graph_counts(FmsSolver *solver, long long *captures, long long *replays)
    states on this batch - fused kernel with gates resident or streamed, 128 or
    256 threads per fibre, 64 or 512 steps per launch; with allow_split also the
    split kernels launched eagerly or replayed as hipGraphs of 8 or 32 steps
-   (models of > 8 states: reaction per fibre or per node, eager or graphs of 8) -
+   (models of > 8 states: reaction per fibre with 256 or 64 threads, or per node,
+   eager or graphs of 8) -
    over the n_steps records of `steps` (4 doubles each, as
    fms_advance_schedule), restores the states, stimulation state and spike
    counters after every trial and keeps the fastest schedule. candidate_ms
@@ -204,6 +208,7 @@ typedef struct {
   double ms_per_step;
   int n_candidates;
   int reaction_per_fibre;
+  int reaction_threads;
 } FmsTuneResult;
 int fms_autotune(FmsSolver *solver, const double *steps, int n_steps, int n0D, int allow_split,
                  FmsTuneResult *result, double *candidate_ms, int capacity);

@@ -20,7 +20,7 @@ FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
 FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
 FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES, FMS_TUNE_ONLY_IF_STIMULATED = 1, 2, 3
 FMS_TUNE_THREADS, FMS_TUNE_GRAPH_STEPS, FMS_TUNE_REACTION_PER_FIBRE, FMS_TUNE_PROFILE_KERNELS = 4, 5, 6, 7
-FMS_TUNE_PAIR_HALVES = 8
+FMS_TUNE_PAIR_HALVES, FMS_TUNE_REACTION_THREADS = 8, 9
 FMS_KERNEL_CLASSES = ("reaction", "diffusion", "fused", "other", "reaction_pair")
 
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -38,6 +38,7 @@ class FmsTuneResult(ctypes.Structure):
         ("ms_per_step", ctypes.c_double),
         ("n_candidates", ctypes.c_int),
         ("reaction_per_fibre", ctypes.c_int),
+        ("reaction_threads", ctypes.c_int),
     ]
 
 

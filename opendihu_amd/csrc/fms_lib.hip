@@ -117,6 +117,7 @@ struct FmsSolver {
   int tune_graph = 0;
   int tune_reaction_fibre = 1;   // split kernels, large models: one workgroup per fibre for the reaction
   int tune_pair = 1;   // split kernels: the two reaction halves between two solves in one launch
+  int tune_reaction_threads = 256;   // split kernels, per-fibre reaction: threads per fibre workgroup (64 or 256)
 
   // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): HIP events
   // around every launch on the solver's stream, waited for one by one
@@ -289,6 +290,19 @@ int upload_steps(FmsSolver *s, const double *steps, int K) {
   return 0;
 }
 
+// the per-fibre reaction of a large model with the workgroup size of the schedule
+template <class G, class E, bool PAIR>
+void launch_reaction_fibre(FmsSolver *s, int q, int half, int n0D, const fms::DevSpikes &spk) {
+  if (s->tune_reaction_threads == 64)
+    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 64, PAIR>), dim3(s->F), dim3(64), 0, s->stream,
+                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+                       s->vstim, s->d_order, spk);
+  else
+    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256, PAIR>), dim3(s->F), dim3(256), 0, s->stream,
+                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+                       s->vstim, s->d_order, spk);
+}
+
 template <class G, class E>
 int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   KernelTimer timer(s, FMS_KERNEL_REACTION);
@@ -296,9 +310,7 @@ int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   const long long blocks = (s->N + threads - 1) / threads;
   const fms::DevSpikes off{};
   if (Model::NS > 8 && !store_alg && s->tune_reaction_fibre) {
-    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256>), dim3(s->F), dim3(256), 0, s->stream, s->fibers(),
-                       s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D, s->vstim,
-                       s->d_order, off);
+    launch_reaction_fibre<G, E, false>(s, q, half, n0D, off);
   } else if (store_alg)
     hipLaunchKernelGGL((fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
                        s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
@@ -319,9 +331,7 @@ int launch_reaction_pair(FmsSolver *s, int q, int n0D) {
   KernelTimer timer(s, FMS_KERNEL_REACTION_PAIR);
   const fms::DevSpikes spk = dev_spikes(s);
   if (Model::NS > 8 && s->tune_reaction_fibre) {
-    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256, true>), dim3(s->F), dim3(256), 0, s->stream,
-                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), 1, n0D,
-                       s->vstim, s->d_order, spk);
+    launch_reaction_fibre<G, E, true>(s, q, 1, n0D, spk);
   } else {
     const int threads = 256;
     const long long blocks = (s->N + threads - 1) / threads;
@@ -412,14 +422,14 @@ std::vector<unsigned char> graph_signature(const FmsSolver *s, int gsteps, int n
   struct {
     const void *vm, *gates, *flags, *cursor, *cA, *dA, *vprev, *sfirst, *scount;
     double vstim, spike_th;
-    int spikes, n0D, gsteps, thomas, scheme, nmax, g_is_double, e_is_approx, pair, per_fibre;
+    int spikes, n0D, gsteps, thomas, scheme, nmax, g_is_double, e_is_approx, pair, per_fibre, rthreads;
   } k;
   memset(&k, 0, sizeof(k));
   k.vm = s->d_vm; k.gates = s->d_gates; k.flags = s->d_flags; k.cursor = s->d_cursor;
   k.cA = s->d_cA; k.dA = s->d_dA; k.vprev = s->d_vprev; k.sfirst = s->d_sfirst; k.scount = s->d_scount;
   k.vstim = s->vstim; k.spike_th = s->spike_th; k.spikes = s->spikes; k.n0D = n0D; k.gsteps = gsteps;
   k.thomas = s->thomas; k.scheme = s->scheme; k.nmax = s->nmax;
-  k.pair = s->tune_pair; k.per_fibre = s->tune_reaction_fibre;
+  k.pair = s->tune_pair; k.per_fibre = s->tune_reaction_fibre; k.rthreads = s->tune_reaction_threads;
   k.g_is_double = std::is_same<G, double>::value;
   k.e_is_approx = std::is_same<E, fms::ExpApprox>::value;
   std::vector<unsigned char> sig(sizeof(fb) + sizeof(st) + sizeof(k));
@@ -1085,6 +1095,10 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "reaction per fibre is 0 or 1");
       s->tune_reaction_fibre = value;
       return 0;
+    case FMS_TUNE_REACTION_THREADS:
+      if (value != 64 && value != 256) return fail(FMS_ERR_ARGUMENT, "reaction threads per fibre are 64 or 256");
+      s->tune_reaction_threads = value;
+      return 0;
     case FMS_TUNE_PAIR_HALVES:
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "paired reaction halves is 0 or 1");
       s->tune_pair = value;
@@ -1468,7 +1482,7 @@ void snapshot_free(Snapshot &sn) {
 }
 
 struct Schedule {
-  int kernels, chunk, resident, threads, graph, per_fibre;
+  int kernels, chunk, resident, threads, graph, per_fibre, rthreads;
 };
 
 int apply_schedule(FmsSolver *s, const Schedule &c) {
@@ -1478,6 +1492,7 @@ int apply_schedule(FmsSolver *s, const Schedule &c) {
   s->tune_threads = c.threads;
   s->tune_graph = c.graph;
   s->tune_reaction_fibre = c.per_fibre;
+  s->tune_reaction_threads = c.rthreads;
   if (c.kernels == FMS_KERNELS_SPLIT && s->thomas == FMS_THOMAS_SERIAL && !s->d_cA) {
     if (dalloc(&s->d_cA, (size_t)s->N) || dalloc(&s->d_dA, (size_t)s->N)) return FMS_ERR_HIP;
   }
@@ -1490,29 +1505,31 @@ std::vector<Schedule> candidates(const FmsSolver *s, int allow_split) {
   std::vector<Schedule> v;
   const bool small = Model::NS <= 8 && s->nmax <= 2048;
   const bool fused_ok = !(s->nmax > 2048 && s->thomas == FMS_THOMAS_SERIAL) && s->nmax <= 4096;
-  const int pf = s->tune_reaction_fibre;
+  const int pf = s->tune_reaction_fibre, rt = s->tune_reaction_threads;
   if (fused_ok) {
     if (small) {
       for (int chunk : {64, 512}) {
-        v.push_back({FMS_KERNELS_FUSED, chunk, 1, 256, 0, pf});
-        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0, pf});
+        v.push_back({FMS_KERNELS_FUSED, chunk, 1, 256, 0, pf, rt});
+        v.push_back({FMS_KERNELS_FUSED, chunk, 0, 256, 0, pf, rt});
         // the partitioned solve's elimination order depends on the threads per
         // fibre, so 128-thread schedules are bit-identical only with the serial solve
         if (s->thomas == FMS_THOMAS_SERIAL && s->nmax <= 1024) {
-          v.push_back({FMS_KERNELS_FUSED, chunk, 1, 128, 0, pf});
-          v.push_back({FMS_KERNELS_FUSED, chunk, 0, 128, 0, pf});
+          v.push_back({FMS_KERNELS_FUSED, chunk, 1, 128, 0, pf, rt});
+          v.push_back({FMS_KERNELS_FUSED, chunk, 0, 128, 0, pf, rt});
         }
       }
     } else {
-      v.push_back({FMS_KERNELS_FUSED, 512, 0, 256, 0, pf});
+      v.push_back({FMS_KERNELS_FUSED, 512, 0, 256, 0, pf, rt});
     }
   }
   if (allow_split || !fused_ok) {
     if (Model::NS > 8) {
-      for (int fibre : {1, 0})
-        for (int g : {0, 8}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, fibre});
+      // reaction per fibre (workgroups of 256 or 64 threads) or per node
+      for (int r : {256, 64})
+        for (int g : {0, 8}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, 1, r});
+      for (int g : {0, 8}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, 0, rt});
     } else {
-      for (int g : {0, 8, 32}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, pf});
+      for (int g : {0, 8, 32}) v.push_back({FMS_KERNELS_SPLIT, s->tune_chunk, s->tune_resident, 256, g, pf, rt});
     }
   }
   return v;
@@ -1529,7 +1546,7 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
   const std::vector<Schedule> cand = candidates(s, allow_split);
   if (cand.empty()) return fail(FMS_ERR_UNSUPPORTED, "no schedule fits this batch");
   const Schedule keep{s->kernels, s->tune_chunk, s->tune_resident, s->tune_threads, s->tune_graph,
-                      s->tune_reaction_fibre};
+                      s->tune_reaction_fibre, s->tune_reaction_threads};
   Snapshot sn;
   int rc = snapshot_take(s, sn);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1576,6 +1593,7 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
     result->threads = cand[best].threads;
     result->graph_steps = cand[best].graph;
     result->reaction_per_fibre = cand[best].per_fibre;
+    result->reaction_threads = cand[best].rthreads;
     result->ms_per_step = best_ms / n_steps;
     result->n_candidates = (int)cand.size();
   }

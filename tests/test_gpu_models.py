@@ -53,9 +53,10 @@ def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=
     mesh, stim = c2(n_fibers=F, n_nodes=n, length_cm=(n - 1) * 0.01, dt=dt, end_time=steps * dt, rate_hz=4000.0)
     per_node = kernels.startswith("split-per-node")
     unpaired = kernels.endswith("-unpaired")
+    threads = 64 if "-64" in kernels else 256
     kernels = "split" if kernels.startswith("split") else kernels
     g = FiberBatch(**mesh, scheme="IE", kernels=kernels, thomas=thomas, precision=precision, library=lib)
-    g.set_tuning(reaction_per_fibre=0 if per_node else 1, pair_halves=0 if unpaired else 1)
+    g.set_tuning(reaction_per_fibre=0 if per_node else 1, pair_halves=0 if unpaired else 1, reaction_threads=threads)
     o = OracleBatch(**mesh, scheme="IE", model=SHORTEN, threads=8)
     for b in (g, o):
         b.set_firing_table_stimulation(**stim)
@@ -64,7 +65,8 @@ def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=
     return g, o
 
 
-@pytest.mark.parametrize("kernels", ["split", "split-unpaired", "split-per-node", "split-per-node-unpaired", "fused"])
+@pytest.mark.parametrize("kernels", ["split", "split-unpaired", "split-64", "split-64-unpaired", "split-per-node",
+                                     "split-per-node-unpaired", "fused"])
 def test_shorten_bit_exact(kernels):
     g, o = _shorten_pair(kernels, "serial")
     for k in range(g.n_states):
