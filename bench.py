@@ -50,7 +50,9 @@ SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 # HBM bytes per node and launch of the dominant kernel, measured with rocprofv3
 # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB device copy):
 # profiles/k_reaction_fibre_pair_shorten56_1024x1481.md
-MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction_pair"): 928.4, ("c2", "reaction"): 925.8}
+# (workload, kernel class, threads per fibre of the per-fibre reaction)
+MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction_pair", 64): 1025.0, ("c2", "reaction_pair", 256): 928.4,
+                                    ("c2", "reaction", 256): 925.8}
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
@@ -356,7 +358,8 @@ def main(argv=None):
         # and write the states once; unit = one node-half
         halves = 2 if kclass == "reaction_pair" else 1
         per_fibre = (schedule or {}).get("reaction_per_fibre", 1)
-        kernel = "k_reaction_fibre" if g.n_states > 8 and per_fibre else "k_reaction"
+        rt = (schedule or {}).get("reaction_threads", 256)
+        kernel = f"k_reaction_fibre<{rt} threads>" if g.n_states > 8 and per_fibre else "k_reaction"
         if halves == 2:
             kernel += " (PAIR: two reaction halves per launch)"
         units_per_launch = n_total * halves
@@ -369,7 +372,8 @@ def main(argv=None):
         b_unit = bytes_per_node_step(g.n_states, "fp64")
     achieved_tflops = f_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e12
     achieved_gbs = b_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    traffic_per_node = MEASURED_TRAFFIC_PER_NODE_LAUNCH.get((args.workload, kclass))
+    rthreads = (schedule or {}).get("reaction_threads", 256)
+    traffic_per_node = MEASURED_TRAFFIC_PER_NODE_LAUNCH.get((args.workload, kclass, rthreads))
     kernel_times = {k: dict(total_ms=v[0], launches=v[1]) for k, v in times.items()}
     g.close()
 
