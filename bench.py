@@ -16,8 +16,9 @@ The JSON line carries:
                    op count, each + - * / exp log = 1) / mean launch time from HIP
                    events on the solver's stream; "hbm" holds the same launch
                    against the HBM roof with B_alg bytes per node-step
-  cpu_baseline     the FP64 CPU oracle (the reference's vc-path restatement) on the
-                   host cores, over a bounded sample of the same workload
+  cpu_baseline     the FP64 CPU oracle (the reference's vc-path restatement, four
+                   nodes per vector operation as Vc's double_v on AVX2) on the host
+                   cores, over a bounded sample of the same workload
   mixed            the FP32-state / FP64-Vm form on the same workload
   workloads        batched uneven HH under the autotuned schedule (FP64, mixed,
                    spike-time error, CPU oracle), the adjoint's gradient cost
@@ -146,7 +147,8 @@ def cpu_baseline(mesh, stim, scheme, model, dt, budget_s=12.0):
         pass
     return dict(value=n_total * steps / el, unit="fiber-node-steps/s", cores=threads, kind="port",
                 sample=f"all {len(mesh['n_nodes'])} fibres ({n_total} nodes) x {steps} split steps "
-                       f"({el:.1f} s, FP64 oracle, {threads} OpenMP threads on {cpu})")
+                       f"({el:.1f} s, FP64 oracle, four nodes per AVX2 vector operation, {threads} OpenMP "
+                       f"threads on {cpu})")
 
 
 def _spike_shift(first_a, first_b, n_steps):

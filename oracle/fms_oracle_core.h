@@ -34,6 +34,10 @@
  *   static void model_rhs_e(...)                          libm exp/log
  *   static const double model_y0[NS];                      initial states
  *   MODEL_NAME                                              string literal
+ * and optionally FMSO_MODEL_RHS_V4, the name of a four-lane form of model_rhs_a
+ * over fmso_v4 (declared below). The reaction then runs four nodes per vector
+ * operation, as the reference's Vc code runs W nodes per double_v
+ * (GEN/03_generator_vc.cpp:1017-1130), each lane with the scalar operations.
  * Compile with -ffp-contract=off so every + - * / rounds as written.
  */
 #ifndef FMS_ORACLE_CORE_H
@@ -81,6 +85,53 @@ static inline double logarithmic_approx(double x)
   result -= (1 - x / exponential_approx(result));
   return result;
 }
+
+/* four lanes of doubles (GCC vector extension): element-wise IEEE operations,
+   the W = 4 double_v of the reference's Vc build on AVX2 */
+typedef double fmso_v4 __attribute__((vector_size(32)));
+
+static inline fmso_v4 exponential_approx_v4(fmso_v4 x)
+{
+  x = 1.0 + x / 1024.;
+  for (int i = 0; i < 10; i++)
+    x *= x;
+  return x;
+}
+static inline fmso_v4 pow3_v4(fmso_v4 x) { return x * (x * x); }
+static inline fmso_v4 pow4_v4(fmso_v4 x) { fmso_v4 s = x * x; return s * s; }
+
+/* lane masks of a comparison and the lane select m ? a : b */
+typedef __typeof__((fmso_v4){0} > (fmso_v4){0}) fmso_v4m;
+static inline fmso_v4 fmso_vsel(fmso_v4m m, fmso_v4 a, fmso_v4 b)
+{
+  return (fmso_v4)((m & (fmso_v4m)a) | (~m & (fmso_v4m)b));
+}
+
+static inline fmso_v4 logarithmic_approx_v4(fmso_v4 x)
+{
+  /* logarithmic_approx per lane: the three Taylor branches selected by lane */
+  fmso_v4 t1 = x - 1;
+  fmso_v4 t1_2 = t1 * t1;
+  fmso_v4 t1_4 = t1_2 * t1_2;
+  fmso_v4 t1_6 = t1_4 * t1_2;
+  fmso_v4 result1 = t1 - 0.5 * t1_2 + 1. / 3 * t1_2 * t1 - 0.25 * t1_4 + 0.2 * t1_4 * t1 - 1. / 6 * t1_6;
+  fmso_v4 t3 = x - 3;
+  fmso_v4 t3_2 = t3 * t3;
+  fmso_v4 t3_4 = t3_2 * t3_2;
+  fmso_v4 result3 = 1.0986122886681098 + 1. / 3 * t3 - 0.05555555555555555 * t3_2 + 0.012345679012345678 * t3_2 * t3 - 0.0030864197530864196 * t3_4;
+  fmso_v4 t9 = x - 9;
+  fmso_v4 t9_2 = t9 * t9;
+  fmso_v4 t9_4 = t9_2 * t9_2;
+  fmso_v4 result9 = 2.1972245773362196 + 1. / 9 * t9 - 0.006172839506172839 * t9_2 + 0.0004572473708276177 * t9_2 * t9 - 3.8103947568968146e-05 * t9_4;
+  fmso_v4 result = fmso_vsel((x >= 2) & (x < 6), result3, result1);
+  result = fmso_vsel(x >= 6, result9, result);
+  result -= (1 - x / exponential_approx_v4(result));
+  return result;
+}
+#ifdef FMSO_MODEL_RHS_V4
+static inline void FMSO_MODEL_RHS_V4(const fmso_v4 *y, const fmso_v4 *p, double t, fmso_v4 *r,
+                                     fmso_v4 *a);
+#endif
 
 /* integer powers, built as the generator builds them: powN = pow(N/2)(pow(N-N/2)) */
 static inline double pow2_(double x) { return x * x; }
@@ -228,7 +279,35 @@ static void fmso_compute0D_fiber(const fmso_problem *p, fmso_state *s, int f, lo
     const int on = flags[j] & 1;
     if (p->stim_mode == FMSO_STIM_CURRENT && NP > 0)
       prm_on[p->stim_param] = on ? p->stim_amplitude[f] : prm[p->stim_param];
-    for (int i = 0; i < n; i++) {
+    int i = 0;
+#ifdef FMSO_MODEL_RHS_V4
+    /* models with a four-lane right-hand side: groups of four nodes, each lane
+       the scalar Heun step's operations (approximate exponential only) */
+    if (p->exp_mode == FMSO_EXP_APPROX) {
+      for (; i + 4 <= n; i += 4) {
+        fmso_v4 y[NS], r[NS], ys[NS], rs[NS], a[NA > 0 ? NA : 1], as[NA > 0 ? NA : 1], pl[NP > 0 ? NP : 1];
+        int clamp[4];
+        const double *pp[4];
+        for (int k = 0; k < NS; k++) memcpy(&y[k], s->states + (long)k * ntot + off + i, sizeof(fmso_v4));
+        for (int l = 0; l < 4; l++) {
+          const int in_set = i + l >= p->stim_begin[f] && i + l < p->stim_end[f];
+          clamp[l] = p->stim_mode == FMSO_STIM_FIRING_TABLE && on && in_set;
+          pp[l] = (p->stim_mode == FMSO_STIM_CURRENT && in_set) ? prm_on : prm;
+        }
+        for (int k = 0; k < NP; k++) pl[k] = (fmso_v4){pp[0][k], pp[1][k], pp[2][k], pp[3][k]};
+        FMSO_MODEL_RHS_V4(y, pl, t, r, a);
+        for (int k = 0; k < NS; k++) ys[k] = y[k] + dt * r[k];
+        for (int l = 0; l < 4; l++)
+          if (clamp[l]) ys[0][l] = p->value_for_stimulated_point;
+        FMSO_MODEL_RHS_V4(ys, pl, t, rs, as);
+        for (int k = 0; k < NS; k++) y[k] += 0.5 * dt * (r[k] + rs[k]);
+        for (int l = 0; l < 4; l++)
+          if (clamp[l]) y[0][l] = p->value_for_stimulated_point;
+        for (int k = 0; k < NS; k++) memcpy(s->states + (long)k * ntot + off + i, &y[k], sizeof(fmso_v4));
+      }
+    }
+#endif
+    for (; i < n; i++) {
       double y[NS];
       for (int k = 0; k < NS; k++) y[k] = s->states[(long)k * ntot + off + i];
       const int in_set = i >= p->stim_begin[f] && i < p->stim_end[f];

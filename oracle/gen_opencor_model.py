@@ -12,7 +12,8 @@ parametersUsedAsAlgebraic entries come first, then parametersUsedAsConstant).
 The generated source contains the model file's text, so it is written to
 oracle/_gen (not versioned) and compiled to oracle/build/libfmso_<name>.so.
 
-usage: python gen_opencor_model.py MODEL.c NAME "alg,..." "const,..."
+usage: python gen_opencor_model.py MODEL.c NAME "alg,..." "const,..." [MARCH [scalar]]
+("scalar": without the four-lane function, one node at a time)
 """
 import os
 import re
@@ -22,7 +23,29 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def main(path, name, p_alg, p_const, march="x86-64-v3"):
+def lane_selects(line):
+    """(C ? A : B) -> SEL_((C), (A), (B)) for the four-lane function (the
+    reference's Vc generator turns the same conditionals into lane selects)."""
+    while "?" in line:
+        q = line.rindex("?")
+        depth, lo = 0, q - 1
+        while lo >= 0 and not (line[lo] == "(" and depth == 0):
+            depth += {")": 1, "(": -1}.get(line[lo], 0)
+            lo -= 1
+        depth, colon, hi = 0, None, q + 1
+        while hi < len(line) and not (line[hi] == ")" and depth == 0):
+            depth += {"(": 1, ")": -1}.get(line[hi], 0)
+            if line[hi] == ":" and depth == 0:
+                colon = hi
+            hi += 1
+        if lo < 0 or hi >= len(line) or colon is None:
+            raise ValueError(f"conditional outside parentheses: {line}")
+        line = (line[:lo] + f"SEL_(({line[lo + 1:q]}), ({line[q + 1:colon]}), ({line[colon + 1:hi]}))"
+                + line[hi + 1:])
+    return line
+
+
+def main(path, name, p_alg, p_const, march="x86-64-v3", scalar_only=False):
     text = open(path).read()
     n_alg = int(re.search(r"There are a total of (\d+) entries in the algebraic", text).group(1))
     n_states = int(re.search(r"There are a total of (\d+) entries in each of the rate and state", text).group(1))
@@ -64,6 +87,9 @@ def main(path, name, p_alg, p_const, march="x86-64-v3"):
 static const double model_y0[NS] = {{{", ".join(y0.get(i, "0") for i in range(n_states))}}};
 static void model_rhs_a(const double *STATES, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
 static void model_rhs_e(const double *STATES, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
+#ifndef FMSO_SCALAR_ONLY
+#define FMSO_MODEL_RHS_V4 model_rhs_a_v4
+#endif
 #include "fms_oracle_core.h"
 
 static double CONSTANTS[{n_const}];
@@ -110,6 +136,46 @@ static void model_rhs_e(const double *STATES, const double *P_, double VOI, doub
 }}
 #undef pow
 
+#ifdef FMSO_MODEL_RHS_V4
+/* the same lines over four nodes per operation: states, parameters, rates and
+   algebraics are four-lane vectors, CONSTANTS stay scalars (broadcast, as Vc
+   broadcasts them), and each function picks its scalar or lane form by type */
+static fmso_v4 ipow_v4_(fmso_v4 x, int n)
+{{
+  if (n == 1) return x;
+  if (n == 2) return x * x;
+  int e0 = n / 2, e1 = n - n / 2;
+  if (e0 == 1) return x * (ipow_v4_(x, e1));
+  return ipow_v4_(ipow_v4_(x, e1), e0);
+}}
+static fmso_v4 oracle_pow_v4(fmso_v4 x, double y)
+{{
+  if (y == (double)(int)y && y != 0) {{
+    int n = (int)y;
+    return n > 0 ? ipow_v4_(x, n) : 1. / (ipow_v4_(x, -n));
+  }}
+  fmso_v4 r;
+  for (int l = 0; l < 4; l++) r[l] = pow(x[l], y);
+  return r;
+}}
+static inline fmso_v4 v4_id_(fmso_v4 x) {{ return x; }}
+static inline fmso_v4 v4_bc_(double x) {{ return (fmso_v4){{x, x, x, x}}; }}
+static inline fmso_v4 v4_sel_s_(int c, fmso_v4 a, fmso_v4 b) {{ return c ? a : b; }}
+#define BC_(x) _Generic((x), fmso_v4: v4_id_, default: v4_bc_)(x)
+#define SEL_(c, a, b) _Generic((c), int: v4_sel_s_, default: fmso_vsel)((c), BC_(a), BC_(b))
+#define pow(x, y) _Generic((x), fmso_v4: oracle_pow_v4, default: oracle_pow)((x), (y))
+#define exp(x) _Generic((x), fmso_v4: exponential_approx_v4, default: exponential_approx)(x)
+#define log(x) _Generic((x), fmso_v4: logarithmic_approx_v4, default: logarithmic_approx)(x)
+static void model_rhs_a_v4(const fmso_v4 *STATES, const fmso_v4 *P_, double VOI, fmso_v4 *RATES, fmso_v4 *ALGEBRAIC)
+{{
+  (void)VOI; (void)P_;
+{chr(10).join(lane_selects(b) for b in body)}
+}}
+#undef pow
+#undef exp
+#undef log
+#endif
+
 void fmso_default_params(double *p)
 {{
 {chr(10).join(f"  p[{j}] = 0.0;" for j in range(len(p_alg)))}
@@ -118,13 +184,13 @@ void fmso_default_params(double *p)
 """
     os.makedirs(os.path.join(HERE, "_gen"), exist_ok=True)
     os.makedirs(os.path.join(HERE, "build"), exist_ok=True)
-    cfile = os.path.join(HERE, "_gen", f"model_{name}.c")
+    cfile = os.path.join(HERE, "_gen", f"model_{name}{'_scalar' if scalar_only else ''}.c")
     with open(cfile, "w") as f:
         f.write(src)
-    out = os.path.join(HERE, "build", f"libfmso_{name}.so")
+    out = os.path.join(HERE, "build", f"libfmso_{name}{'_scalar' if scalar_only else ''}.so")
     subprocess.run(["gcc", "-O3", f"-march={march}", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
                     "-shared", "-std=gnu11", "-Wno-unused-function", "-Wno-unused-variable", "-I", HERE, "-o", out,
-                    cfile, "-lm"], check=True)
+                    cfile, "-lm"] + (["-DFMSO_SCALAR_ONLY"] if scalar_only else []), check=True)
     return out
 
 
@@ -132,4 +198,5 @@ if __name__ == "__main__":
     pa = [int(x) for x in sys.argv[3].split(",") if x] if len(sys.argv) > 3 else []
     pc = [int(x) for x in sys.argv[4].split(",") if x] if len(sys.argv) > 4 else []
     march = sys.argv[5] if len(sys.argv) > 5 else "x86-64-v3"
-    print(main(sys.argv[1], sys.argv[2], pa, pc, march))
+    scalar = len(sys.argv) > 6 and sys.argv[6] == "scalar"
+    print(main(sys.argv[1], sys.argv[2], pa, pc, march, scalar))

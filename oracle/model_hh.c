@@ -18,8 +18,10 @@
 
 static const double model_y0[NS] = {-75, 0.05, 0.6, 0.325};
 
-#define HH_RHS(NAME, EXPF)                                                                        \
-  static void NAME(const double *y, const double *p, double t, double *r, double *a)             \
+/* T: double, or fmso_v4 (four nodes per operation, each lane the same IEEE
+   operations as the scalar form, as the reference's Vc double_v evaluates W nodes) */
+#define HH_RHS(NAME, T, EXPF, POW3, POW4)                                                         \
+  static inline void NAME(const T *y, const T *p, double t, T *r, T *a)                         \
   {                                                                                               \
     (void)t;                                                                                      \
     const double constant0 = -75;                                                                 \
@@ -27,7 +29,7 @@ static const double model_y0[NS] = {-75, 0.05, 0.6, 0.325};
     const double constant6 = constant0 + 115.000;                                                 \
     const double constant7 = constant0 - 12.0000;                                                 \
     const double constant8 = constant0 + 10.6130;                                                 \
-    const double i_stim = p[0], g_na = p[1], g_k = p[2], g_l = p[3];                             \
+    const T i_stim = p[0], g_na = p[1], g_k = p[2], g_l = p[3];                                  \
     a[1] = (-0.100000 * (y[0] + 50.0000)) / (EXPF(-(y[0] + 50.0000) / 10.0000) - 1.00000);       \
     a[5] = 4.00000 * EXPF(-(y[0] + 75.0000) / 18.0000);                                          \
     r[1] = a[1] * (1.00000 - y[1]) - a[5] * y[1];                                                 \
@@ -37,19 +39,25 @@ static const double model_y0[NS] = {-75, 0.05, 0.6, 0.325};
     a[3] = (-0.0100000 * (y[0] + 65.0000)) / (EXPF(-(y[0] + 65.0000) / 10.0000) - 1.00000);      \
     a[7] = 0.125000 * EXPF((y[0] + 75.0000) / 80.0000);                                          \
     r[3] = a[3] * (1.00000 - y[3]) - a[7] * y[3];                                                 \
-    a[0] = g_na * pow3_(y[1]) * y[2] * (y[0] - constant6);                                        \
-    a[4] = g_k * pow4_(y[3]) * (y[0] - constant7);                                                \
+    a[0] = g_na * POW3(y[1]) * y[2] * (y[0] - constant6);                                         \
+    a[4] = g_k * POW4(y[3]) * (y[0] - constant7);                                                 \
     a[8] = g_l * (y[0] - constant8);                                                              \
     r[0] = -(-i_stim + a[0] + a[4] + a[8]) / constant1;                                           \
   }
 
-static void model_rhs_a(const double *y, const double *p, double t, double *r, double *a);
-static void model_rhs_e(const double *y, const double *p, double t, double *r, double *a);
+static inline void model_rhs_a(const double *y, const double *p, double t, double *r, double *a);
+static inline void model_rhs_e(const double *y, const double *p, double t, double *r, double *a);
+#ifndef FMSO_SCALAR_ONLY
+#define FMSO_MODEL_RHS_V4 model_rhs_a_v4
+#endif
 
 #include "fms_oracle_core.h"
 
-HH_RHS(model_rhs_a, exponential_approx)
-HH_RHS(model_rhs_e, exp)
+HH_RHS(model_rhs_a, double, exponential_approx, pow3_, pow4_)
+HH_RHS(model_rhs_e, double, exp, pow3_, pow4_)
+#ifdef FMSO_MODEL_RHS_V4
+HH_RHS(model_rhs_a_v4, fmso_v4, exponential_approx_v4, pow3_v4, pow4_v4)
+#endif
 
 /* default parameter vector of the model file: i_Stim = 0, g_Na = 120, g_K = 36, g_L = 0.3 */
 void fmso_default_params(double *p)
