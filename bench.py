@@ -22,8 +22,10 @@ The JSON line carries:
   mixed            the FP32-state / FP64-Vm form on the same workload
   workloads        batched uneven HH under the autotuned schedule (FP64, mixed,
                    spike-time error, CPU oracle), the adjoint's gradient cost
-                   against one forward solve, and the large mixed-precision
-                   Shorten batch (16384 uneven fibres) against its FP64 run
+                   against one forward solve, the large mixed-precision
+                   Shorten batch (16384 uneven fibres) against its FP64 run, and
+                   the 57-state TK2014 model on the default batch (FP64, mixed,
+                   CPU oracle)
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1] [--no-extras]
 """
 from __future__ import annotations
@@ -299,6 +301,38 @@ def bench_c5(steps=200, warmup=20):
                                    after_steps=steps + warmup))
 
 
+def bench_tk2014(cpu=True, steps=1000, warmup=50):
+    """The 57-state / 71-algebraic TK2014 model (converted from the reference's
+    hand-written Vc kernel) on the headline batch: 1024 fibres x 1481 nodes, IE,
+    dt=2.5e-5 ms, 10 motor units at 20 Hz. FP64 and mixed under the autotuned
+    schedule, FP64 CPU oracle on a bounded sample."""
+    from cases import c2
+    from opendihu_amd.fibers import FiberBatch
+    from opendihu_amd.build import LIB_DIR, TK2014_LIBRARY
+    if not os.path.exists(os.path.join(LIB_DIR, TK2014_LIBRARY)):
+        return dict(skipped=f"{TK2014_LIBRARY} not built (the reference checkout is needed at build time)")
+    dt = 2.5e-5
+    mesh, stim = c2()
+    res = {}
+    for precision in ("fp64", "mixed"):
+        g = FiberBatch(**mesh, scheme="IE", precision=precision, kernels="fused", thomas="partitioned",
+                       library=TK2014_LIBRARY)
+        g.set_firing_table_stimulation(**stim)
+        sched = g.autotune(step_table(0.0, dt, 16), allow_split=True)
+        wall, _, _ = time_gpu(g, dt, steps, warmup)
+        res[precision] = dict(value=g.n_total * steps / wall, ms_per_step=wall / steps * 1e3, schedule=sched)
+        g.close()
+    out = dict(workload="tk2014_57_1024x1481_IE_dt2.5e-5", nodes=int(mesh["n_nodes"].sum()), steps=steps,
+               warmup=warmup, value_fp64=res["fp64"]["value"], value_mixed=res["mixed"]["value"],
+               ms_per_step_fp64=res["fp64"]["ms_per_step"], ms_per_step_mixed=res["mixed"]["ms_per_step"],
+               schedule_fp64=res["fp64"]["schedule"], schedule_mixed=res["mixed"]["schedule"])
+    if cpu:
+        c = cpu_baseline(mesh, stim, "IE", "tk2014", dt)
+        out["cpu_baseline"] = c
+        out["speedup_fp64_vs_cpu"] = res["fp64"]["value"] / c["value"]
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +345,7 @@ def main(argv=None):
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-HH (autotuned), adjoint and large mixed-precision Shorten lines")
-    ap.add_argument("--extras", default="c3,c4,c5", help="comma list of extra workloads to run")
+    ap.add_argument("--extras", default="c3,c4,c5,tk2014", help="comma list of extra workloads to run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -403,6 +437,8 @@ def main(argv=None):
                 extras[name] = bench_c4()
             elif name == "c5":
                 extras[name] = bench_c5()
+            elif name == "tk2014":
+                extras[name] = bench_tk2014(cpu=not args.no_cpu_baseline)
             else:
                 raise SystemExit(f"unknown extra workload {name}")
             extras[name]["bench_wall_s"] = time.perf_counter() - t0

@@ -59,11 +59,14 @@ def _run(cmd: list[str]) -> None:
 
 
 def compile_library(out_name: str, model_header: str | None = None, model_type: str | None = None,
-                    extra_include: str | None = None, n_states: int = 4) -> str:
-    """Compile fms_lib.hip for one model into ``LIB_DIR/out_name``."""
+                    extra_include: str | None = None, n_states: int = 4, ilp_schedule: bool | None = None) -> str:
+    """Compile fms_lib.hip for one model into ``LIB_DIR/out_name``. ilp_schedule:
+    the machine scheduler for ILP (None: for models of more than 8 states)."""
     os.makedirs(LIB_DIR, exist_ok=True)
     out = os.path.join(LIB_DIR, out_name)
-    flags = HIP_FLAGS + (LARGE_MODEL_FLAGS if n_states > LARGE_MODEL_STATES else [])
+    if ilp_schedule is None:
+        ilp_schedule = n_states > LARGE_MODEL_STATES
+    flags = HIP_FLAGS + (LARGE_MODEL_FLAGS if ilp_schedule else [])
     cmd = [HIPCC, *flags, f"-I{INCLUDE}", f"-I{CSRC}"]
     if extra_include:
         cmd.append(f"-I{extra_include}")
@@ -81,7 +84,7 @@ def build_hh() -> str:
 
 
 def build_model_library(model_c_file: str, parameters_used_as_algebraic=(), parameters_used_as_constant=(),
-                        name: str | None = None, lib_name: str | None = None) -> str:
+                        name: str | None = None, lib_name: str | None = None, ilp_schedule: bool | None = None) -> str:
     """Generate HIP code for an OpenCOR C model file and compile its library."""
     from .cellml import generate_hip_model
 
@@ -100,7 +103,7 @@ def build_model_library(model_c_file: str, parameters_used_as_algebraic=(), para
     from .cellml import parse_opencor_c
     n_states = parse_opencor_c(text).n_states
     return compile_library(lib_name or f"libfms_{base}.so", header_name, type_name, extra_include=GEN_DIR,
-                           n_states=n_states)
+                           n_states=n_states, ilp_schedule=ilp_schedule)
 
 
 # CellML models shipped with the reference's tests (testing/unit_testing/input),
@@ -118,6 +121,33 @@ REFERENCE_MODELS = [
 SHORTEN_LIBRARY = "libfms_shorten_ocallaghan_davidson_soboleva_2007.so"
 GEN_HH_LIBRARY = "libfms_gen_hodgkin_huxley_1952.so"
 
+# Models the reference ships only as hand-written Vc kernels, converted to the
+# OpenCOR form by opendihu_amd.cellml.vc_kernel (no parameters: the kernels
+# take none, stimulation clamps Vm):
+#   TK2014, 57 states / 71 algebraics (the kernel's new_slow_TK_2014_12_08).
+#   Built with the default machine scheduler: under the ILP scheduler its
+#   per-fibre reaction kernel (k_reaction_fibre) computed wrong states that
+#   changed from run to run (NaN in one run, rates off by 100 % in another) on
+#   gfx950 with hipcc of ROCm 7.2, while the default scheduler's build is
+#   bit-identical to the oracle in every kernel variant
+#   (tests/test_gpu_models.py).
+VC_KERNEL_MODELS = [
+    ("core/src/specialized_solver/fast_monodomain_solver/tk2014.cpp.disabled", "tk2014", "libfms_tk2014.so", False),
+]
+TK2014_LIBRARY = "libfms_tk2014.so"
+
+
+def build_vc_kernel_library(kernel_file: str, name: str, lib_name: str, ilp_schedule: bool | None = None) -> str:
+    """Convert a hand-written Vc kernel to the OpenCOR form and compile its library."""
+    from .cellml import convert_vc_kernel
+    os.makedirs(GEN_DIR, exist_ok=True)
+    with open(kernel_file) as f:
+        text = convert_vc_kernel(f.read(), name)
+    path = os.path.join(GEN_DIR, f"{name}_opencor.c")
+    with open(path, "w") as f:
+        f.write(text)
+    return build_model_library(path, [], [], name, lib_name, ilp_schedule)
+
 
 def build_reference_models(reference_root: str) -> list[str]:
     """Libraries for the CellML models shipped in the reference's test inputs
@@ -125,8 +155,11 @@ def build_reference_models(reference_root: str) -> list[str]:
     from concurrent.futures import ThreadPoolExecutor
     jobs = [(os.path.join(reference_root, rel), pa, pc, name, lib)
             for rel, name, pa, pc, lib in REFERENCE_MODELS if os.path.exists(os.path.join(reference_root, rel))]
-    with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+    vc_jobs = [(os.path.join(reference_root, rel), name, lib, ilp)
+               for rel, name, lib, ilp in VC_KERNEL_MODELS if os.path.exists(os.path.join(reference_root, rel))]
+    with ThreadPoolExecutor(max_workers=max(1, len(jobs) + len(vc_jobs))) as ex:
         futs = [ex.submit(build_model_library, path, pa, pc, name, lib) for path, pa, pc, name, lib in jobs]
+        futs += [ex.submit(build_vc_kernel_library, path, name, lib, ilp) for path, name, lib, ilp in vc_jobs]
         return [f.result() for f in futs]
 
 

@@ -7,5 +7,6 @@ This package does the same ahead of time for the HIP fibre kernels.
 """
 from .opencor import OpenCorModel, parse_opencor_c
 from .codegen import generate_hip_model
+from .vc_kernel import convert_vc_kernel
 
-__all__ = ["OpenCorModel", "parse_opencor_c", "generate_hip_model"]
+__all__ = ["OpenCorModel", "parse_opencor_c", "generate_hip_model", "convert_vc_kernel"]

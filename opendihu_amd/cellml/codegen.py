@@ -11,7 +11,8 @@ source_code_generator/03_generator_vc.cpp:937-1016); with -ffp-contract=off on
 both sides the FP64 results agree bit for bit. Rewrites, as the reference's
 preprocessCode (03_generator_vc.cpp:13-302) applies them:
 
-* ``exp(`` -> ``E::exp(``, ``log(`` -> ``E::log(`` (approximation or device libm)
+* ``exp(`` -> ``E::exp(``, ``log(`` -> ``E::log(`` (approximation or device libm);
+  ``log_accurate(`` (models converted by vc_kernel.py) -> the portable accurate log
 * ``pow(x, n)`` with an integer literal n -> ``pow<n>(x)`` / ``powReciprocal<|n|>(x)``
   (recursive products, 03_generator_vc.cpp:541-611); other pow -> ``::pow``
 * ``CONSTANTS[k]`` -> the constant, or a parameter when k is in
@@ -31,7 +32,9 @@ from .opencor import OpenCorModel, parse_opencor_c
 _TOKEN_RE = re.compile(r"\s*(?:(\d+\.\d*(?:[eE][-+]?\d+)?|\d+(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|"
                        r"([A-Za-z_]\w*)|(&&|\|\||<=|>=|==|!=|[-+*/()?:<>,\[\]!]))")
 
-_FUNCS = {"exp": "E::exp", "log": "E::log", "fabs": "fabs", "sqrt": "sqrt"}
+_FUNCS = {"exp": "E::exp", "log": "E::log", "fabs": "fabs", "sqrt": "sqrt",
+          # accurate log of models converted from hand-written kernels (vc_kernel.py)
+          "log_accurate": "log_accurate"}
 
 
 def _tokenize(expr: str) -> list[str]:
@@ -171,7 +174,7 @@ def count_ops(model: OpenCorModel, parameters_used_as_algebraic=()) -> dict:
                 c["div"] += 1
             elif t == "exp":
                 c["exp"] += 1
-            elif t == "log":
+            elif t in ("log", "log_accurate"):
                 c["log"] += 1
             elif t in ("<", ">", "<=", ">=", "==", "!="):
                 c["cmp"] += 1

@@ -113,6 +113,44 @@ struct ExpLibm {
   static FMS_HD FD log(FD x) { return FD(::log(x.v)); }
 };
 
+// Natural logarithm from IEEE double operations only (no library call beyond
+// the exact frexp), so that a C build with the same operations returns the
+// same bits on the host. Models whose source calls an accurate log outside the
+// generator's approximation policy use it: the hand-written TK2014 kernel calls
+// Vc::log (core/src/specialized_solver/fast_monodomain_solver/
+// tk2014.cpp.disabled:242,269).
+//   x = m 2^k, m in [sqrt(1/2), sqrt(2)); s = (m - 1)/(m + 1) (m - 1 exact);
+//   log m = 2 atanh s = 2s + s z P(z), z = s^2 <= 0.0295,
+//   P(z) = sum_{n=1..10} 2 z^(n-1) / (2n + 1) (truncation below 2^-54 relative);
+//   log x = k ln2_hi + (log m + k ln2_lo), ln2_hi with 32 significant bits.
+FMS_HD double log_accurate(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / (x - x);   // -inf, or NaN for x < 0 / NaN
+  if (x == HUGE_VAL) return x;
+  int k;
+  double m = frexp(x, &k);   // x = m 2^k, m in [0.5, 1)
+  if (m < 0.7071067811865476) {
+    m = 2.0 * m;
+    k -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 0.09523809523809523;
+  p = p * z + 0.10526315789473684;
+  p = p * z + 0.11764705882352941;
+  p = p * z + 0.13333333333333333;
+  p = p * z + 0.15384615384615385;
+  p = p * z + 0.18181818181818182;
+  p = p * z + 0.2222222222222222;
+  p = p * z + 0.2857142857142857;
+  p = p * z + 0.4;
+  p = p * z + 0.6666666666666666;
+  const double log_m = 2.0 * s + s * z * p;
+  const double dk = (double)k;
+  return dk * 0.6931471803691238 + (log_m + dk * 1.9082149292705877e-10);
+}
+FMS_HD FD log_accurate(FD x) { return FD(log_accurate(x.v)); }
+FMS_HD float log_accurate(float x) { return (float)log_accurate((double)x); }
+
 template <class T> FMS_HD T pow2(T x) { return x * x; }
 template <class T> FMS_HD T pow3(T x) { return x * (pow2(x)); }
 template <class T> FMS_HD T pow4(T x) { return pow2(pow2(x)); }

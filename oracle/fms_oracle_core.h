@@ -86,6 +86,39 @@ static inline double logarithmic_approx(double x)
   return result;
 }
 
+/* accurate natural logarithm for the hand-written TK2014 kernel, which calls
+   Vc::log (FMS/tk2014.cpp.disabled:242,269) outside the generator's exp/log
+   approximation: x = m 2^k, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+   log m = 2 atanh s = 2s + s z P(z) with z = s^2 and P the first ten terms
+   2 z^(n-1)/(2n+1) of the atanh series, log x = k ln2_hi + (log m + k ln2_lo).
+   Plain IEEE operations and the exact frexp: no libm rounding enters */
+static inline double fmso_log_accurate(double x)
+{
+  if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / (x - x);
+  if (x == HUGE_VAL) return x;
+  int k;
+  double m = frexp(x, &k);
+  if (m < 0.7071067811865476) {
+    m = 2.0 * m;
+    k -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 0.09523809523809523;
+  p = p * z + 0.10526315789473684;
+  p = p * z + 0.11764705882352941;
+  p = p * z + 0.13333333333333333;
+  p = p * z + 0.15384615384615385;
+  p = p * z + 0.18181818181818182;
+  p = p * z + 0.2222222222222222;
+  p = p * z + 0.2857142857142857;
+  p = p * z + 0.4;
+  p = p * z + 0.6666666666666666;
+  const double log_m = 2.0 * s + s * z * p;
+  const double dk = (double)k;
+  return dk * 0.6931471803691238 + (log_m + dk * 1.9082149292705877e-10);
+}
+
 /* four lanes of doubles (GCC vector extension): element-wise IEEE operations,
    the W = 4 double_v of the reference's Vc build on AVX2 */
 typedef double fmso_v4 __attribute__((vector_size(32)));
@@ -105,6 +138,21 @@ typedef __typeof__((fmso_v4){0} > (fmso_v4){0}) fmso_v4m;
 static inline fmso_v4 fmso_vsel(fmso_v4m m, fmso_v4 a, fmso_v4 b)
 {
   return (fmso_v4)((m & (fmso_v4m)a) | (~m & (fmso_v4m)b));
+}
+
+/* scalar-or-lane helpers of the four-lane model forms: broadcast a scalar, and
+   c ? a : b with c a scalar condition or a lane mask */
+static inline fmso_v4 fmso_v4_id(fmso_v4 x) { return x; }
+static inline fmso_v4 fmso_v4_bc(double x) { return (fmso_v4){x, x, x, x}; }
+static inline fmso_v4 fmso_v4_sel_s(int c, fmso_v4 a, fmso_v4 b) { return c ? a : b; }
+#define FMSO_BC(x) _Generic((x), fmso_v4: fmso_v4_id, default: fmso_v4_bc)(x)
+#define FMSO_SEL(c, a, b) _Generic((c), int: fmso_v4_sel_s, default: fmso_vsel)((c), FMSO_BC(a), FMSO_BC(b))
+
+static inline fmso_v4 fmso_log_accurate_v4(fmso_v4 x)
+{
+  fmso_v4 r;
+  for (int l = 0; l < 4; l++) r[l] = fmso_log_accurate(x[l]);
+  return r;
 }
 
 static inline fmso_v4 logarithmic_approx_v4(fmso_v4 x)
@@ -594,5 +642,11 @@ int fmso_schedule(double rc_end, double rc_width, double strang_end, double stra
 }
 
 const char *fmso_model_name(void) { return MODEL_NAME; }
+
+/* the accurate logarithm over an array (checked against the host's log) */
+void fmso_log_accurate_n(const double *x, double *y, long n)
+{
+  for (long i = 0; i < n; i++) y[i] = fmso_log_accurate(x[i]);
+}
 
 #endif /* FMS_ORACLE_CORE_H */

@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def lane_selects(line):
-    """(C ? A : B) -> SEL_((C), (A), (B)) for the four-lane function (the
+    """(C ? A : B) -> FMSO_SEL((C), (A), (B)) for the four-lane function (the
     reference's Vc generator turns the same conditionals into lane selects)."""
     while "?" in line:
         q = line.rindex("?")
@@ -40,7 +40,7 @@ def lane_selects(line):
             hi += 1
         if lo < 0 or hi >= len(line) or colon is None:
             raise ValueError(f"conditional outside parentheses: {line}")
-        line = (line[:lo] + f"SEL_(({line[lo + 1:q]}), ({line[q + 1:colon]}), ({line[colon + 1:hi]}))"
+        line = (line[:lo] + f"FMSO_SEL(({line[lo + 1:q]}), ({line[q + 1:colon]}), ({line[colon + 1:hi]}))"
                 + line[hi + 1:])
     return line
 
@@ -158,11 +158,6 @@ static fmso_v4 oracle_pow_v4(fmso_v4 x, double y)
   for (int l = 0; l < 4; l++) r[l] = pow(x[l], y);
   return r;
 }}
-static inline fmso_v4 v4_id_(fmso_v4 x) {{ return x; }}
-static inline fmso_v4 v4_bc_(double x) {{ return (fmso_v4){{x, x, x, x}}; }}
-static inline fmso_v4 v4_sel_s_(int c, fmso_v4 a, fmso_v4 b) {{ return c ? a : b; }}
-#define BC_(x) _Generic((x), fmso_v4: v4_id_, default: v4_bc_)(x)
-#define SEL_(c, a, b) _Generic((c), int: v4_sel_s_, default: fmso_vsel)((c), BC_(a), BC_(b))
 #define pow(x, y) _Generic((x), fmso_v4: oracle_pow_v4, default: oracle_pow)((x), (y))
 #define exp(x) _Generic((x), fmso_v4: exponential_approx_v4, default: exponential_approx)(x)
 #define log(x) _Generic((x), fmso_v4: logarithmic_approx_v4, default: logarithmic_approx)(x)

@@ -4,7 +4,10 @@
   bit-identical to the hand-written HH library (same expression order);
 * the 56-state Shorten library must be bit-identical to the oracle's Shorten
   model with the serial Thomas solve, and within 1e-10 (relative L2) with the
-  partitioned solve.
+  partitioned solve;
+* the 57-state TK2014 library (converted from the reference's hand-written Vc
+  kernel) must be bit-identical to the oracle's TK2014 model, which takes the
+  kernel's lines by another route.
 The generated libraries are built from the reference's CellML inputs where that
 checkout exists (build()); the tests skip when the library was not built.
 """
@@ -45,11 +48,11 @@ def test_generated_hh_equals_handwritten():
     np.testing.assert_array_equal(runs[0], runs[1])
 
 
-def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=2e-4):
+def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=2e-4, model=SHORTEN):
     from opendihu_amd.fibers import FiberBatch
-    lib = _lib_or_skip(f"libfms_{SHORTEN}.so")
-    if not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}.so")):
-        pytest.skip("oracle Shorten model not built")
+    lib = _lib_or_skip(f"libfms_{model}.so")
+    if not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{model}.so")):
+        pytest.skip(f"oracle model {model} not built")
     mesh, stim = c2(n_fibers=F, n_nodes=n, length_cm=(n - 1) * 0.01, dt=dt, end_time=steps * dt, rate_hz=4000.0)
     per_node = kernels.startswith("split-per-node")
     unpaired = kernels.endswith("-unpaired")
@@ -57,7 +60,7 @@ def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=
     kernels = "split" if kernels.startswith("split") else kernels
     g = FiberBatch(**mesh, scheme="IE", kernels=kernels, thomas=thomas, precision=precision, library=lib)
     g.set_tuning(reaction_per_fibre=0 if per_node else 1, pair_halves=0 if unpaired else 1, reaction_threads=threads)
-    o = OracleBatch(**mesh, scheme="IE", model=SHORTEN, threads=8)
+    o = OracleBatch(**mesh, scheme="IE", model=model, threads=8)
     for b in (g, o):
         b.set_firing_table_stimulation(**stim)
         b.enable_spike_detection(0.0)
@@ -74,6 +77,24 @@ def test_shorten_bit_exact(kernels):
     assert (o.spikes()[1] > 0).sum() > 0, "stimulus must trigger action potentials"
     for a, b in zip(g.spikes(), o.spikes()):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("kernels", ["split", "split-64-unpaired", "split-per-node", "fused"])
+def test_tk2014_bit_exact(kernels):
+    g, o = _shorten_pair(kernels, "serial", F=4, n=201, steps=1200, dt=1e-4, model="tk2014")
+    for k in range(g.n_states):
+        np.testing.assert_array_equal(g.get_state(k), o.get_state(k), err_msg=f"state {k}")
+    assert (o.spikes()[1] > 0).sum() > 0, "stimulus must trigger action potentials"
+    for a, b in zip(g.spikes(), o.spikes()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_tk2014_partitioned():
+    g, o = _shorten_pair("split", "partitioned", F=4, n=201, steps=1200, dt=1e-4, model="tk2014")
+    err = _rel(g.vm(), o.vm())
+    print(f"tk2014 split partitioned: Vm rel L2 err {err:.3e}")
+    assert err <= 1e-10
+    np.testing.assert_array_equal(g.spikes()[0], o.spikes()[0])
 
 
 def test_shorten_partitioned():
@@ -113,15 +134,15 @@ def _random_points(lib_name, oracle_model, n=4000, seed=0):
     return L, g, o, P, _dp, Y, Pm
 
 
-@pytest.mark.parametrize("which", ["hh", SHORTEN])
+@pytest.mark.parametrize("which", ["hh", SHORTEN, "tk2014"])
 @pytest.mark.parametrize("exp_mode", ["approx", "libm"])
 def test_device_rhs_and_heun_match_oracle(which, exp_mode):
     """the compiled model evaluated on the device at random states: rates and ten
     Heun steps bit-identical to the oracle with the approximate exponential; with
     the math library exp/log (device libm vs glibc) within 1e-10 relative"""
-    lib = "libfms_hodgkin_huxley_1952.so" if which == "hh" else _lib_or_skip(f"libfms_{SHORTEN}.so")
-    if which != "hh" and not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}.so")):
-        pytest.skip("oracle Shorten model not built")
+    lib = "libfms_hodgkin_huxley_1952.so" if which == "hh" else _lib_or_skip(f"libfms_{which}.so")
+    if which != "hh" and not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{which}.so")):
+        pytest.skip(f"oracle model {which} not built")
     L, g, o, P, _dp, Y, Pm = _random_points(lib, which)
     mode = 0 if exp_mode == "approx" else 1
     n = Y.shape[1]
