@@ -26,7 +26,7 @@ The JSON line carries:
                    Shorten batch (16384 uneven fibres) against its FP64 run, and
                    the 57-state TK2014 model on the default batch (FP64, mixed,
                    CPU oracle)
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1] [--no-extras]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1|tk2014] [--no-extras]
 """
 from __future__ import annotations
 
@@ -86,6 +86,16 @@ def make_workload(name: str, precision: str, kernels: str):
                     scheme="ImplicitEuler", dt_split_ms=2.5e-5, motor_units=10, firing_hz=20,
                     stimulation="firing table, Vm clamp (reference vc semantics, W=4)")
         return g, mesh, stim, desc, 2.5e-5, "IE", SHORTEN
+    if name == "tk2014":
+        from opendihu_amd.build import TK2014_LIBRARY
+        mesh, stim = c2()
+        g = FiberBatch(**mesh, scheme="IE", precision=precision, kernels=kernels, thomas="partitioned",
+                       library=TK2014_LIBRARY)
+        g.set_firing_table_stimulation(**stim)
+        desc = dict(workload="tk2014_57_1024x1481_IE_dt2.5e-5", model="tk2014 (57 states, 71 algebraics)",
+                    fibers=1024, nodes_per_fiber=1481, scheme="ImplicitEuler", dt_split_ms=2.5e-5, motor_units=10,
+                    firing_hz=20, stimulation="firing table, Vm clamp (reference vc semantics, W=4)")
+        return g, mesh, stim, desc, 2.5e-5, "IE", "tk2014"
     if name == "c3":
         mesh, stim = uneven(4096, seed=3)
         g = FiberBatch(**mesh, scheme="CN", precision=precision, kernels=kernels, thomas="partitioned",
@@ -338,7 +348,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c1", "tk2014"])
     ap.add_argument("--kernels", default="auto", choices=["auto", "fused", "split"],
                     help="auto: the autotuner's fastest bit-identical schedule for this batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
