@@ -2,7 +2,8 @@
 a whole batch on the GPU is checked fibre by fibre against the oracle run on a
 few sampled fibres only (with the fibre-local stimulation ranges and motor
 units they have in the full batch): the Shorten batch of 1024 fibres x 1481
-nodes, the 4096 uneven HH fibres and the 16384-fibre mixed-precision batch.
+nodes, the same batch with the 57-state TK2014 model, the 4096 uneven HH fibres
+and the 16384-fibre mixed-precision batch.
 """
 import numpy as np
 import pytest
@@ -61,6 +62,36 @@ def test_shorten_1024x1481_sampled_fibres(thomas):
             np.testing.assert_array_equal(gate0[s], og0[so])
         else:
             assert _rel(vm[s], ovm[so]) <= 1e-10
+        np.testing.assert_array_equal(first[s], ofirst[so])
+    assert (ofirst >= 0).sum() > 0, "the sampled fibres must fire"
+
+
+def test_tk2014_1024x1481_sampled_fibres():
+    import os
+    from opendihu_amd.fibers import FiberBatch
+    from opendihu_amd.build import LIB_DIR, TK2014_LIBRARY
+    from oracle_binding import ORACLE_BUILD
+    if not os.path.exists(os.path.join(LIB_DIR, TK2014_LIBRARY)) or \
+            not os.path.exists(os.path.join(ORACLE_BUILD, "libfmso_tk2014.so")):
+        pytest.skip("TK2014 library or oracle model not built (needs the reference checkout at build time)")
+    dt, steps = 2.5e-5, 300
+    mesh, stim = c2(rate_hz=4000.0)
+    fibres = [0, 511, 1023]
+    g = FiberBatch(**mesh, scheme="IE", kernels="split", thomas="serial", library=TK2014_LIBRARY)
+    g.set_firing_table_stimulation(**stim)
+    g.enable_spike_detection(-40.0)
+    g.advance(0.0, dt, steps)
+    vm, gate, first = g.vm(), g.get_state(18), g.spikes()[0]
+    g.close()
+    sm, ss = _subset(mesh, stim, fibres)
+    o = OracleBatch(**sm, scheme="IE", model="tk2014", threads=8)
+    o.set_firing_table_stimulation(**ss)
+    o.enable_spike_detection(-40.0)
+    o.advance(0.0, dt, steps)
+    ovm, og, ofirst = o.vm(), o.get_state(18), o.spikes()[0]
+    for s, so in zip(_slices(mesh, fibres), _slices(sm, range(len(fibres)))):
+        np.testing.assert_array_equal(vm[s], ovm[so])
+        np.testing.assert_array_equal(gate[s], og[so])
         np.testing.assert_array_equal(first[s], ofirst[so])
     assert (ofirst >= 0).sum() > 0, "the sampled fibres must fire"
 

@@ -105,6 +105,18 @@ def test_shorten_partitioned():
     np.testing.assert_array_equal(g.spikes()[0], o.spikes()[0])
 
 
+def test_tk2014_mixed_precision():
+    g, o = _shorten_pair("fused", "partitioned", F=4, n=201, steps=1200, dt=1e-4, precision="mixed", model="tk2014")
+    gf, of = g.spikes()[0], o.spikes()[0]
+    fired = (gf > 0) | (of > 0)
+    assert fired.any(), "stimulus must trigger action potentials"
+    a = np.where(gf > 0, gf, 10**9)
+    b = np.where(of > 0, of, 10**9)
+    shift = int(np.max(np.abs(a[fired] - b[fired])))
+    print(f"tk2014 mixed: spike-time error {shift} steps, Vm max abs err {np.max(np.abs(g.vm() - o.vm())):.3e} mV")
+    assert shift <= 1
+
+
 def test_shorten_mixed_precision():
     g, o = _shorten_pair("fused", "partitioned", precision="mixed")
     gf, of = g.spikes()[0], o.spikes()[0]
