@@ -267,6 +267,9 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   double l[NPT][4], g[NPT][3];
   int slot[NPT];
   double pfacc = 0;
+  // the fibre's running prefactor gradient, held by thread 0 for the segment:
+  // the same additions in the same order as adding to pf_bar[f] every step
+  double pftotal = tid == 0 ? pf_bar[f] : 0.0;
   // lanes past the fibre's end shadow its last node: they compute, never store
 #pragma unroll
   for (int q = 0; q < NPT; q++) {
@@ -338,7 +341,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
       double sum = red[0];
 #pragma unroll
       for (int q = 1; q < T / 64; q++) sum += red[q];
-      pf_bar[f] += sum;
+      pftotal += sum;
     }
     // (every read of lw and lv above precedes that barrier, and the next step
     // writes red only after the barrier that follows its first loop, so thread
@@ -351,6 +354,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
       reverse_half<E>(st, f, inset, y0[q], p, t, dt0, n0D, l[q], g[q]);
     }
   }
+  if (tid == 0) pf_bar[f] = pftotal;
 #pragma unroll
   for (int q = 0; q < NPT; q++) {
     const int i = tid + q * T;
