@@ -47,15 +47,13 @@ FP64_PEAK_TFLOPS = 78.6
 
 SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 
-# HBM bytes per node and launch unit (reaction: one half step; fused: one split
-# step) of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000, factors from a 1 GiB copy); the summaries are in profiles/
 # HBM bytes per node and launch of the dominant kernel, measured with rocprofv3
-# (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB device copy):
-# profiles/k_reaction_fibre_pair_shorten56_1024x1481.md
-# (workload, kernel class, threads per fibre of the per-fibre reaction)
+# (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB device copy), keyed by
+# (workload, kernel class, threads per fibre of the per-fibre reaction):
+# profiles/k_reaction_fibre_pair_shorten56_1024x1481.md,
+# profiles/k_reaction_fibre_pair_tk2014_1024x1481.md
 MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction_pair", 64): 1025.0, ("c2", "reaction_pair", 256): 928.4,
-                                    ("c2", "reaction", 256): 925.8}
+                                    ("c2", "reaction", 256): 925.8, ("tk2014", "reaction_pair", 64): 1042.6}
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
