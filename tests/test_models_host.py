@@ -29,6 +29,7 @@ CSRC = os.path.join(REPO, "opendihu_amd", "csrc")
 REFERENCE = os.environ.get("OPENDIHU_REFERENCE", "/root/reference")
 TK_KERNEL = os.path.join(REFERENCE, "core/src/specialized_solver/fast_monodomain_solver/tk2014.cpp.disabled")
 SHORTEN_C = os.path.join(REFERENCE, "testing/unit_testing/input/shorten_ocallaghan_davidson_soboleva_2007.c")
+HH_C = os.path.join(REFERENCE, "testing/unit_testing/input/hodgkin_huxley_1952.c")
 SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 
 DRIVER = r"""
@@ -77,14 +78,16 @@ def _oracle_or_skip(model):
     return oracle_library(model)
 
 
-@pytest.mark.parametrize("model", ["tk2014", SHORTEN])
+@pytest.mark.parametrize("model", ["tk2014", SHORTEN, "hodgkin_huxley_gen"])
 def test_generated_model_on_host_matches_oracle(tmp_path, model):
     if model == "tk2014":
         text, p_alg, p_const = _tk_text(), (), ()
     else:
-        if not os.path.exists(SHORTEN_C):
-            pytest.skip("the reference's Shorten model file is not present")
-        text, p_alg, p_const = open(SHORTEN_C).read(), (32,), (65,)
+        path = SHORTEN_C if model == SHORTEN else HH_C
+        if not os.path.exists(path):
+            pytest.skip("the reference's model file is not present")
+        text = open(path).read()
+        p_alg, p_const = ((32,), (65,)) if model == SHORTEN else ((), (2, 3, 4, 5))
     o = _oracle_or_skip(model)
     lib = _host_model(tmp_path, text, p_alg, p_const)
     ns = lib.ns()
