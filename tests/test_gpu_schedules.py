@@ -65,6 +65,37 @@ def test_graph_replay_equals_eager(thomas, pair):
     assert runs["eager"][3]["replays"] == 0
 
 
+@pytest.mark.parametrize("reaction_threads", [64, 256])
+def test_graph_replay_equals_eager_large_model(reaction_threads):
+    """the 57-state TK2014 library (per-fibre reaction kernel, paired halves):
+    graph replays bit-identical to eager launches across advance calls"""
+    import os
+    from opendihu_amd.build import LIB_DIR, TK2014_LIBRARY
+    if not os.path.exists(os.path.join(LIB_DIR, TK2014_LIBRARY)):
+        pytest.skip("TK2014 library not built (needs the reference checkout at build time)")
+    mesh, _ = uneven(8, lo=200, hi=600, seed=11)
+    calls = (37, 8, 50, 3, 300)
+    runs = {}
+    for mode in ("eager", "graph"):
+        g = _batch(mesh, scheme="IE", kernels="split", thomas="partitioned", library=TK2014_LIBRARY)
+        g.set_tuning(reaction_per_fibre=1, reaction_threads=reaction_threads, graph_steps=8 if mode == "graph" else 0)
+        _firing(g, mesh)
+        g.enable_spike_detection(-40.0)
+        t, dt, per_call = 0.0, 2.5e-5, []
+        for k in calls:
+            g.advance(t, dt, k)
+            t += k * dt
+            per_call.append(g.get_states())
+        runs[mode] = (per_call, g.spikes(), g.graph_counts())
+        g.close()
+    for c, (a, b) in enumerate(zip(runs["eager"][0], runs["graph"][0])):
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, b, err_msg=f"advance call {c}")
+    for a, b in zip(runs["eager"][1], runs["graph"][1]):
+        np.testing.assert_array_equal(a, b)
+    assert runs["graph"][2]["replays"] == sum((k - 1) // 8 for k in calls)
+
+
 def test_graph_follows_stimulation_and_spike_changes():
     """new stimulation values are read from device memory by the same graph;
     switching the spike detector on changes the launch sequence and recaptures"""
