@@ -227,8 +227,21 @@ typedef struct {
   /* spike detector (extension): upward crossing of the threshold */
   double spike_threshold;
   /* onlyComputeIfHasBeenStimulated (compute.tpp:201-205), applied per fibre:
-     the reaction of a fibre runs from the Heun step of its first stimulation on */
+     the reaction of a fibre runs from the Heun step of its first stimulation on
+     (fmso_advance_vc applies it in the reference's point-buffer order) */
   int only_if_stimulated;
+  /* algebraicsForTransfer: model algebraic numbers whose values the last reaction
+     half of a call stores (the second right-hand-side evaluation of its last
+     Heun step, GEN/03_generator_vc.cpp:1131-1162; storeAlgebraicsForTransfer on
+     the last split step only, compute.tpp:116-125, 192). Algebraics that are
+     parameters (parametersUsedAsAlgebraic) have no case in the generated switch
+     and keep their previous value. */
+  int n_alg_transfer;
+  const int *alg_transfer_idx;   /* [n_alg_transfer] */
+  /* reference point-buffer sweep (fmso_advance_vc only) */
+  int vc_width;                  /* W: nodes per point buffer (Vc::double_v::size()) */
+  const int *nmj;                /* [n_fibers] fiberStimulationPointIndex */
+  int equilibrium_skip;          /* disableComputationWhenStatesAreCloseToEquilibrium */
 } fmso_problem;
 
 typedef struct {
@@ -245,7 +258,29 @@ typedef struct {
   double *stim_log_time;         /* [stim_log_capacity] onset times */
   int *stim_log_fiber;           /* [stim_log_capacity] */
   int stim_log_capacity;
+  double *alg_transfer;          /* [n_alg_transfer][n_total] or NULL */
+  /* fmso_advance_vc only */
+  unsigned char *vc_eq;          /* [n_buffers] fiberPointBuffersStatesAreCloseToEquilibrium_:
+                                    0 inactive, 1 neighbor_is_active, 2 active */
+  int64_t *vc_counters;          /* [4]: nFiberPointBufferStatesCloseToEquilibrium_, buffer Heun
+                                    steps computed, skipped at equilibrium, skipped before the
+                                    fibre's first stimulation */
+  double *vc_pad;                /* [NS][W]: lanes of the last point buffer past the last node */
 } fmso_state;
+
+/* algebraics that are parameters (parametersUsedAsAlgebraic): the model
+   translation unit lists them in MODEL_PARAM_ALGEBRAICS */
+static inline int fmso_alg_is_param(int k)
+{
+#ifdef MODEL_PARAM_ALGEBRAICS
+  static const int list[] = {MODEL_PARAM_ALGEBRAICS};
+  for (unsigned q = 0; q < sizeof(list) / sizeof(list[0]); q++)
+    if (list[q] == k) return 1;
+#else
+  (void)k;
+#endif
+  return 0;
+}
 
 static long fmso_total_nodes(const fmso_problem *p)
 {
@@ -302,8 +337,11 @@ static inline int fmso_current_on(const fmso_problem *p, int f, double t)
 
 /* ---- one Heun step of one node, GEN/03_generator_vc.cpp:1017-1130 ---- */
 
+/* alg_out (may be NULL) receives the algebraics of the second evaluation, the
+   values the generated code copies to algebraicsForTransfer
+   (GEN/03_generator_vc.cpp:1131-1162) */
 static inline void fmso_heun_node(double *y, const double *prm, double t, double dt,
-                                  int clamp, double vstim, int exp_mode)
+                                  int clamp, double vstim, int exp_mode, double *alg_out)
 {
   double r[NS], ys[NS], rs[NS], a[NA > 0 ? NA : 1], as[NA > 0 ? NA : 1];
   if (exp_mode == FMSO_EXP_APPROX) model_rhs_a(y, prm, t, r, a); else model_rhs_e(y, prm, t, r, a);
@@ -312,11 +350,23 @@ static inline void fmso_heun_node(double *y, const double *prm, double t, double
   if (exp_mode == FMSO_EXP_APPROX) model_rhs_a(ys, prm, t, rs, as); else model_rhs_e(ys, prm, t, rs, as);
   for (int k = 0; k < NS; k++) y[k] += 0.5 * dt * (r[k] + rs[k]);
   if (clamp) y[0] = vstim;
+  if (alg_out)
+    for (int k = 0; k < NA; k++) alg_out[k] = as[k];
 }
 
-/* reaction half step of one fibre: n0D Heun steps starting at t0 */
+/* the requested algebraics of node i (global) from a full algebraic vector */
+static inline void fmso_store_alg(const fmso_problem *p, fmso_state *s, long i, long ntot, const double *alg)
+{
+  for (int q = 0; q < p->n_alg_transfer; q++) {
+    const int k = p->alg_transfer_idx[q];
+    if (!fmso_alg_is_param(k)) s->alg_transfer[(long)q * ntot + i] = alg[k];
+  }
+}
+
+/* reaction half step of one fibre: n0D Heun steps starting at t0; store_alg:
+   the last Heun step stores the algebraics for transfer */
 static void fmso_compute0D_fiber(const fmso_problem *p, fmso_state *s, int f, long off, long ntot,
-                                 double t0, double dt, int n0D, const unsigned char *flags)
+                                 double t0, double dt, int n0D, const unsigned char *flags, int store_alg)
 {
   const int n = p->n_nodes[f];
   double prm[NP > 0 ? NP : 1], prm_on[NP > 0 ? NP : 1];
@@ -325,6 +375,7 @@ static void fmso_compute0D_fiber(const fmso_problem *p, fmso_state *s, int f, lo
     const double t = t0 + j * dt;
     if (!(flags[j] & 2)) continue;
     const int on = flags[j] & 1;
+    const int store = store_alg && j == n0D - 1 && s->alg_transfer && p->n_alg_transfer > 0;
     if (p->stim_mode == FMSO_STIM_CURRENT && NP > 0)
       prm_on[p->stim_param] = on ? p->stim_amplitude[f] : prm[p->stim_param];
     int i = 0;
@@ -352,17 +403,24 @@ static void fmso_compute0D_fiber(const fmso_problem *p, fmso_state *s, int f, lo
         for (int l = 0; l < 4; l++)
           if (clamp[l]) y[0][l] = p->value_for_stimulated_point;
         for (int k = 0; k < NS; k++) memcpy(s->states + (long)k * ntot + off + i, &y[k], sizeof(fmso_v4));
+        if (store)
+          for (int l = 0; l < 4; l++) {
+            double al[NA > 0 ? NA : 1];
+            for (int k = 0; k < NA; k++) al[k] = as[k][l];
+            fmso_store_alg(p, s, off + i + l, ntot, al);
+          }
       }
     }
 #endif
     for (; i < n; i++) {
-      double y[NS];
+      double y[NS], al[NA > 0 ? NA : 1];
       for (int k = 0; k < NS; k++) y[k] = s->states[(long)k * ntot + off + i];
       const int in_set = i >= p->stim_begin[f] && i < p->stim_end[f];
       const int clamp = p->stim_mode == FMSO_STIM_FIRING_TABLE && on && in_set;
       const double *pp = (p->stim_mode == FMSO_STIM_CURRENT && in_set) ? prm_on : prm;
-      fmso_heun_node(y, pp, t, dt, clamp, p->value_for_stimulated_point, p->exp_mode);
+      fmso_heun_node(y, pp, t, dt, clamp, p->value_for_stimulated_point, p->exp_mode, store ? al : NULL);
       for (int k = 0; k < NS; k++) s->states[(long)k * ntot + off + i] = y[k];
+      if (store) fmso_store_alg(p, s, off + i, ntot, al);
     }
   }
 }
@@ -500,11 +558,11 @@ int fmso_advance(const fmso_problem *p, fmso_state *s, double t0, double dts, in
       if (p->spike_threshold == p->spike_threshold)
         memcpy(vb, s->states + offs[f], sizeof(double) * n);
       fmso_fiber_flags(p, s, f, t, dt0D, n0D, flags);
-      fmso_compute0D_fiber(p, s, f, offs[f], ntot, t, dt0D, n0D, flags);
+      fmso_compute0D_fiber(p, s, f, offs[f], ntot, t, dt0D, n0D, flags, 0);
       fmso_thomas_fiber(s->states + offs[f], n, p->elem_len + eoffs[f], p->prefactor[f], dt1D,
                         p->scheme, p->mass, cA, dA);
       fmso_fiber_flags(p, s, f, tmid, dt0D, n0D, flags);
-      fmso_compute0D_fiber(p, s, f, offs[f], ntot, tmid, dt0D, n0D, flags);
+      fmso_compute0D_fiber(p, s, f, offs[f], ntot, tmid, dt0D, n0D, flags, step == n_split - 1);
       if (p->spike_threshold == p->spike_threshold) {
         fmso_state sl = *s;
         sl.step_counter = s->step_counter + step + 1;
@@ -516,6 +574,243 @@ int fmso_advance(const fmso_problem *p, fmso_state *s, double t0, double dts, in
   s->step_counter += n_split;
   free(offs); free(eoffs);
   return 0;
+}
+
+/* ---- the reference's point-buffer sweep, FMS/fast_monodomain_solver_compute.tpp:131-254 ----
+
+   The nodes of all fibres, concatenated, are grouped into point buffers of W
+   consecutive nodes (the last one padded), a buffer may straddle two fibres.
+   compute0D walks the buffers in order; buffer p belongs to fibre
+   (int)(p * (W / n_nodes[0])) (the uniform-length assumption, :158-160) and is
+   the fibre's "centre" buffer when (unsigned long)(nmj - indexInFiber) < W
+   (:163-166). Only the centre buffer runs the fibre's stimulation state machine
+   (:188-192), which also sets fiberHasBeenStimulated_ (:595-598); stimulated
+   centre buffers clamp their lanes 0..min(3,W)-1 (GEN/03_generator_vc.cpp:
+   1032-1038, 1124-1130). Per buffer and Heun step, the equilibrium skip
+   (:196-200, 765-797) and then onlyComputeIfHasBeenStimulated (:202-206) may
+   skip the step; after its Heun steps the buffer's equilibrium state is
+   updated (:209, 629-760). compute1D is skipped when the counter of inactive
+   buffers equals the number of buffers (:261-265).
+
+   Lanes past the last node carry the model's initial states and are computed
+   like the others (no diffusion); the equilibrium check looks at the lanes that
+   hold nodes only. Firing-table stimulation only; single-threaded. */
+
+enum { FMSO_EQ_INACTIVE = 0, FMSO_EQ_NEIGHBOR_IS_ACTIVE = 1, FMSO_EQ_ACTIVE = 2 };
+
+/* isEquilibriumAccelerationCurrentPointDisabled, compute.tpp:763-797 */
+static int fmso_vc_eq_disabled(const fmso_problem *p, fmso_state *s, int stim, long b, long nbuf)
+{
+  if (!p->equilibrium_skip) return 0;
+  unsigned char *eq = s->vc_eq;
+  if (stim) {
+    eq[b] = FMSO_EQ_ACTIVE;
+    if (b > 0 && eq[b - 1] == FMSO_EQ_INACTIVE) {
+      s->vc_counters[0]--;
+      eq[b - 1] = FMSO_EQ_NEIGHBOR_IS_ACTIVE;
+    }
+    if (b < nbuf - 1 && eq[b + 1] == FMSO_EQ_INACTIVE) {
+      s->vc_counters[0]--;
+      eq[b + 1] = FMSO_EQ_NEIGHBOR_IS_ACTIVE;
+    }
+  }
+  return eq[b] == FMSO_EQ_INACTIVE;
+}
+
+/* equilibriumAccelerationUpdate, compute.tpp:629-760; prev/now: [NS][W] values
+   before and after the buffer's Heun steps, nlanes: lanes that hold nodes */
+static void fmso_vc_eq_update(const fmso_problem *p, fmso_state *s, const double *prev, const double *now,
+                              int W, int nlanes, long b, long nbuf)
+{
+  if (!p->equilibrium_skip) return;
+  unsigned char *eq = s->vc_eq;
+  int at_eq = 1;
+  if (eq[b] != FMSO_EQ_INACTIVE) {
+    for (int k = 0; k < NS && at_eq; k++) {
+      double vmin = now[k * W];
+      for (int l = 1; l < nlanes; l++) vmin = now[k * W + l] < vmin ? now[k * W + l] : vmin;
+      const int absolute = fabs(vmin) < 1e-11;   /* Vc::min over the lanes, then fabs */
+      double change = 0.0;
+      for (int l = 0; l < nlanes; l++) {
+        const double nv = now[k * W + l], ov = prev[k * W + l];
+        const double c = absolute ? fabs(nv - ov) : fabs((nv - ov) / nv);
+        change = c > change ? c : change;
+      }
+      if (change > 1e-6) at_eq = 0;
+    }
+  }
+  if (eq[b] == FMSO_EQ_NEIGHBOR_IS_ACTIVE && at_eq) {
+    if (b > 0 && eq[b - 1] != FMSO_EQ_ACTIVE && eq[b] != FMSO_EQ_INACTIVE) {
+      eq[b] = FMSO_EQ_INACTIVE;
+      s->vc_counters[0]++;
+    }
+    if (b < nbuf - 1 && eq[b + 1] != FMSO_EQ_ACTIVE && eq[b] != FMSO_EQ_INACTIVE) {
+      eq[b] = FMSO_EQ_INACTIVE;
+      s->vc_counters[0]++;
+    }
+  }
+  if (eq[b] == FMSO_EQ_ACTIVE || eq[b] == FMSO_EQ_NEIGHBOR_IS_ACTIVE) {
+    if (at_eq) {
+      eq[b] = FMSO_EQ_INACTIVE;
+      s->vc_counters[0]++;
+    } else {
+      eq[b] = FMSO_EQ_ACTIVE;
+      if (b > 0 && eq[b - 1] == FMSO_EQ_INACTIVE) {
+        s->vc_counters[0]--;
+        eq[b - 1] = FMSO_EQ_NEIGHBOR_IS_ACTIVE;
+      }
+      if (b < nbuf - 1 && eq[b + 1] == FMSO_EQ_INACTIVE) {
+        s->vc_counters[0]--;
+        eq[b + 1] = FMSO_EQ_NEIGHBOR_IS_ACTIVE;
+      }
+    }
+  }
+}
+
+/* compute0D over the point buffers, compute.tpp:131-216 */
+static void fmso_vc_compute0D(const fmso_problem *p, fmso_state *s, const long *offs, long ntot, double t0,
+                              double dt, int n0D, int store_alg)
+{
+  const int W = p->vc_width;
+  const long nbuf = (ntot + W - 1) / W;
+  double *lane = (double *)malloc(sizeof(double) * 2 * NS * W);
+  double *prev = lane + NS * W;
+  if (nbuf > 0) {
+    s->vc_eq[0] = FMSO_EQ_ACTIVE;
+    s->vc_eq[nbuf - 1] = FMSO_EQ_ACTIVE;
+  }
+  const double factor = (double)W / p->n_nodes[0];
+  for (long b = 0; b < nbuf; b++) {
+    int f = (int)(b * factor);
+    f = f < 0 ? 0 : (f >= p->n_fibers ? p->n_fibers - 1 : f);
+    const long index_in_fiber = b * W - offs[f];
+    const int centre = (unsigned long)((long)p->nmj[f] - index_in_fiber) < (unsigned long)W;
+    const int nlanes = (int)(ntot - b * W < W ? ntot - b * W : W);
+    /* gather the buffer: [NS][W] */
+    for (int k = 0; k < NS; k++)
+      for (int l = 0; l < W; l++)
+        lane[k * W + l] = l < nlanes ? s->states[(long)k * ntot + b * W + l] : s->vc_pad[k * W + l];
+    memcpy(prev, lane, sizeof(double) * NS * W);
+    for (int j = 0; j < n0D; j++) {
+      const double t = t0 + j * dt;
+      int stim = 0;
+      if (centre) stim = fmso_stim_check(p, s, f, t);
+      if (fmso_vc_eq_disabled(p, s, stim, b, nbuf)) {
+        s->vc_counters[2]++;
+        continue;
+      }
+      if (p->only_if_stimulated && !s->has_been_stimulated[f]) {
+        s->vc_counters[3]++;
+        continue;
+      }
+      s->vc_counters[1]++;
+      const int store = store_alg && j == n0D - 1 && s->alg_transfer && p->n_alg_transfer > 0;
+      for (int l = 0; l < W; l++) {
+        double y[NS], al[NA > 0 ? NA : 1];
+        /* parameters of the lane's own fibre (fetchFiberData gathers them per node) */
+        int fl = f;
+        if (l < nlanes) {
+          const long g = b * W + l;
+          while (fl + 1 < p->n_fibers && g >= offs[fl + 1]) fl++;
+          while (fl > 0 && g < offs[fl]) fl--;
+        }
+        for (int k = 0; k < NS; k++) y[k] = lane[k * W + l];
+        fmso_heun_node(y, p->params + (long)fl * NP, t, dt, stim && l < (W < 3 ? W : 3),
+                       p->value_for_stimulated_point, p->exp_mode, store ? al : NULL);
+        for (int k = 0; k < NS; k++) lane[k * W + l] = y[k];
+        if (store && l < nlanes) fmso_store_alg(p, s, b * W + l, ntot, al);
+      }
+    }
+    for (int k = 0; k < NS; k++)
+      for (int l = 0; l < W; l++) {
+        if (l < nlanes) s->states[(long)k * ntot + b * W + l] = lane[k * W + l];
+        else s->vc_pad[k * W + l] = lane[k * W + l];
+      }
+    fmso_vc_eq_update(p, s, prev, lane, W, nlanes, b, nbuf);
+  }
+  free(lane);
+}
+
+/* computeMonodomain in the reference's order (compute.tpp:110-126) with its
+   point-buffer sweep; s->vc_eq, s->vc_counters and s->vc_pad must be set
+   (vc_eq all FMSO_EQ_ACTIVE, counters 0 and the pad lanes the initial states
+   at the start, initialization.tpp:430-431) */
+int fmso_advance_vc(const fmso_problem *p, fmso_state *s, double t0, double dts, int n_split,
+                    double dt0D, int n0D, double dt1D, int n1D)
+{
+  (void)n1D;
+  if (p->vc_width < 1 || !p->nmj || !s->vc_eq || !s->vc_counters || !s->vc_pad) return -1;
+  if (p->stim_mode == FMSO_STIM_CURRENT) return -1;
+  const long ntot = fmso_total_nodes(p);
+  const long nbuf = (ntot + p->vc_width - 1) / p->vc_width;
+  long *offs = (long *)malloc(sizeof(long) * (p->n_fibers + 1));
+  long *eoffs = (long *)malloc(sizeof(long) * (p->n_fibers + 1));
+  offs[0] = 0; eoffs[0] = 0;
+  int nmax = 2;
+  for (int f = 0; f < p->n_fibers; f++) {
+    offs[f + 1] = offs[f] + p->n_nodes[f];
+    eoffs[f + 1] = eoffs[f] + p->n_nodes[f] - 1;
+    if (p->n_nodes[f] > nmax) nmax = p->n_nodes[f];
+  }
+  double *cA = (double *)malloc(sizeof(double) * nmax);
+  double *dA = (double *)malloc(sizeof(double) * nmax);
+  double *vb = (double *)malloc(sizeof(double) * (ntot > 0 ? ntot : 1));
+  for (int step = 0; step < n_split; step++) {
+    const double t = t0 + step * dts;
+    const double tmid = t + 0.5 * dts;
+    if (p->spike_threshold == p->spike_threshold) memcpy(vb, s->states, sizeof(double) * ntot);
+    fmso_vc_compute0D(p, s, offs, ntot, t, dt0D, n0D, 0);
+    if (s->vc_counters[0] != nbuf)
+      for (int f = 0; f < p->n_fibers; f++)
+        fmso_thomas_fiber(s->states + offs[f], p->n_nodes[f], p->elem_len + eoffs[f], p->prefactor[f], dt1D,
+                          p->scheme, p->mass, cA, dA);
+    fmso_vc_compute0D(p, s, offs, ntot, tmid, dt0D, n0D, step == n_split - 1);
+    if (p->spike_threshold == p->spike_threshold) {
+      fmso_state sl = *s;
+      sl.step_counter = s->step_counter + step + 1;
+      fmso_spikes(p, &sl, 0, ntot, ntot, vb);
+    }
+  }
+  s->step_counter += n_split;
+  free(cA); free(dA); free(vb); free(offs); free(eoffs);
+  return 0;
+}
+
+/* initializeStatesToEquilibrium: classical RK4 of one cell with the model's own
+   (libm) functions until the largest state increment per unit time falls below
+   1e-5; a divergent run (increment > 1e4) restarts from the initial values
+   with half the step (core/src/cellml/03_cellml_adapter.tpp:446-520). Returns
+   the number of iterations, or -1 if max_iter iterations did not converge;
+   *max_increment receives the last largest increment, *dt_out the final step. */
+long fmso_equilibrium(double *u, const double *prm, double dt, long max_iter, double *max_increment, double *dt_out)
+{
+  double prev[NS], k1[NS], k2[NS], k3[NS], k4[NS], u2[NS], u3[NS], u4[NS], a[NA > 0 ? NA : 1];
+  memcpy(prev, u, sizeof(prev));
+  double t = 0.0, incr = 0.0;
+  long it;
+  for (it = 0; it < max_iter; it++) {
+    model_rhs_e(u, prm, t, k1, a);
+    for (int k = 0; k < NS; k++) u2[k] = u[k] + dt / 2. * k1[k];
+    model_rhs_e(u2, prm, t + dt / 2., k2, a);
+    for (int k = 0; k < NS; k++) u3[k] = u[k] + dt / 2. * k2[k];
+    model_rhs_e(u3, prm, t + dt / 2., k3, a);
+    for (int k = 0; k < NS; k++) u4[k] = u[k] + dt * k3[k];
+    model_rhs_e(u4, prm, t + dt, k4, a);
+    incr = 0.0;
+    for (int k = 0; k < NS; k++) {
+      const double inc = (k1[k] + 2 * k2[k] + 2 * k3[k] + k4[k]) / 6.;
+      if (fabs(inc) > incr) incr = fabs(inc);
+      u[k] += dt * inc;
+    }
+    if (incr > 1e4) {
+      dt /= 2.0;
+      memcpy(u, prev, sizeof(prev));
+    }
+    if (incr < 1e-5) break;
+  }
+  if (max_increment) *max_increment = incr;
+  if (dt_out) *dt_out = dt;
+  return incr < 1e-5 ? it : -1;
 }
 
 /* ---- exported single-purpose entry points used by the tests ---- */
@@ -541,7 +836,7 @@ void fmso_rhs(const double *y, const double *prm, double t, double *rates, doubl
 /* n Heun steps of one cell, no diffusion */
 void fmso_heun_cell(double *y, const double *prm, double t0, double dt, int n, int exp_mode)
 {
-  for (int j = 0; j < n; j++) fmso_heun_node(y, prm, t0 + j * dt, dt, 0, 0.0, exp_mode);
+  for (int j = 0; j < n; j++) fmso_heun_node(y, prm, t0 + j * dt, dt, 0, 0.0, exp_mode, NULL);
 }
 
 /* n explicit Euler steps of one cell (used for the reference's CellML known-answer tests) */
@@ -647,6 +942,18 @@ const char *fmso_model_name(void) { return MODEL_NAME; }
 void fmso_log_accurate_n(const double *x, double *y, long n)
 {
   for (long i = 0; i < n; i++) y[i] = fmso_log_accurate(x[i]);
+}
+
+/* the generator's approximate exponential and logarithm over an array (checked
+   against their mathematical definitions and the reference's stated bounds) */
+void fmso_exp_approx_n(const double *x, double *y, long n)
+{
+  for (long i = 0; i < n; i++) y[i] = exponential_approx(x[i]);
+}
+
+void fmso_log_approx_n(const double *x, double *y, long n)
+{
+  for (long i = 0; i < n; i++) y[i] = logarithmic_approx(x[i]);
 }
 
 #endif /* FMS_ORACLE_CORE_H */

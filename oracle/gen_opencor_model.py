@@ -84,6 +84,7 @@ def main(path, name, p_alg, p_const, march="x86-64-v3", scalar_only=False):
 #define NA {n_alg}
 #define NP {np_}
 #define MODEL_NAME "{name}"
+{f"#define MODEL_PARAM_ALGEBRAICS {', '.join(str(k) for k in p_alg)}" if p_alg else ""}
 static const double model_y0[NS] = {{{", ".join(y0.get(i, "0") for i in range(n_states))}}};
 static void model_rhs_a(const double *STATES, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
 static void model_rhs_e(const double *STATES, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);

@@ -2,7 +2,8 @@
 
 Sizes follow the fibre configurations the GPU path is measured on:
   c1     HH, 1 fibre x 1001 nodes (1000 elements, 10 cm), Am=500, Cm=1,
-         sigma=3.828, CN, dt=1e-3 ms, 1200 uA/cm^2 on the central 3 nodes for 0.1 ms
+         sigma=3.828, CN, dt=1e-3 ms, 1200 uA/cm^2 on the central 3 nodes for
+         0.1 ms; Vm0 = -75 mV with the gates at their steady state there
   uneven HH, fibres of U{lo..hi} nodes, per-fibre sigma ~ U(3.0,4.5),
          stimulus node ~ U{0..n-1} and onset ~ U(0,2) ms (current pulses)
 """
@@ -13,13 +14,32 @@ import numpy as np
 AM, CM, SIGMA = 500.0, 1.0, 3.828
 
 
+def hh_rate_constants(v):
+    """(alpha, beta) of the gates m, h, n of the Hodgkin-Huxley model with the
+    resting potential shifted to -75 mV (textbook forms, libm exp)"""
+    v = np.asarray(v, np.float64)
+    am = -0.1 * (v + 50.0) / (np.exp(-(v + 50.0) / 10.0) - 1.0)
+    bm = 4.0 * np.exp(-(v + 75.0) / 18.0)
+    ah = 0.07 * np.exp(-(v + 75.0) / 20.0)
+    bh = 1.0 / (np.exp(-(v + 45.0) / 10.0) + 1.0)
+    an = -0.01 * (v + 65.0) / (np.exp(-(v + 65.0) / 10.0) - 1.0)
+    bn = 0.125 * np.exp((v + 75.0) / 80.0)
+    return (am, bm), (ah, bh), (an, bn)
+
+
+def hh_steady_state_gates(v):
+    """m, h, n = alpha/(alpha+beta) at the membrane potential v"""
+    return tuple(a / (a + b) for a, b in hh_rate_constants(v))
+
+
 def c1(n_elements=1000, length_cm=10.0):
     n = n_elements + 1
     h = np.full(n_elements, length_cm / n_elements)
     pf = SIGMA / (AM * CM)
     mid = n // 2
     stim = dict(stim_begin=mid - 1, stim_end=mid + 2, onset=0.0, duration=0.1, amplitude=1200.0, stim_param=0)
-    return dict(n_nodes=np.array([n], np.int32), element_lengths=h, prefactor=np.array([pf])), stim
+    y0 = np.array([-75.0, *hh_steady_state_gates(-75.0)])
+    return dict(n_nodes=np.array([n], np.int32), element_lengths=h, prefactor=np.array([pf]), initial_states=y0), stim
 
 
 def uneven(n_fibers, lo=400, hi=2000, seed=3, h=0.01):

@@ -32,6 +32,8 @@ class fmso_problem(ctypes.Structure):
         ("stim_onset", _dp), ("stim_duration", _dp), ("stim_amplitude", _dp), ("stim_param", ctypes.c_int),
         ("spike_threshold", ctypes.c_double),
         ("only_if_stimulated", ctypes.c_int),
+        ("n_alg_transfer", ctypes.c_int), ("alg_transfer_idx", _ip),
+        ("vc_width", ctypes.c_int), ("nmj", _ip), ("equilibrium_skip", ctypes.c_int),
     ]
 
 
@@ -41,6 +43,7 @@ class fmso_state(ctypes.Structure):
         ("currently_stimulating", _ucp), ("has_been_stimulated", _ucp), ("first_spike_step", _ip),
         ("spike_count", _ip), ("step_counter", ctypes.c_int64), ("stim_log_count", _ip),
         ("stim_log_time", _dp), ("stim_log_fiber", _ip), ("stim_log_capacity", ctypes.c_int),
+        ("alg_transfer", _dp), ("vc_eq", _ucp), ("vc_counters", ctypes.POINTER(ctypes.c_int64)), ("vc_pad", _dp),
     ]
 
 
@@ -65,6 +68,9 @@ def oracle_library(model: str = "hh") -> ctypes.CDLL:
     lib.fmso_default_params.argtypes = [_dp]
     lib.fmso_schedule.argtypes = [d, d, d, d, d, d, d, d, i, _dp]
     lib.fmso_model_name.restype = ctypes.c_char_p
+    lib.fmso_advance_vc.argtypes = [ctypes.POINTER(fmso_problem), ctypes.POINTER(fmso_state), d, d, i, d, i, d, i]
+    lib.fmso_equilibrium.argtypes = [_dp, _dp, d, ctypes.c_long, _dp, _dp]
+    lib.fmso_equilibrium.restype = ctypes.c_long
     _LIBS[path] = lib
     return lib
 
@@ -77,6 +83,16 @@ def model_info(lib):
     ns, na, np_ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib.fmso_model_info(ctypes.byref(ns), ctypes.byref(na), ctypes.byref(np_))
     return ns.value, na.value, np_.value
+
+
+def equilibrium(lib, y0, params, dt=1e-4, max_iter=10_000_000):
+    """initializeStatesToEquilibrium of one cell: (states, iterations or -1,
+    last largest increment, final step width)"""
+    y = np.ascontiguousarray(y0, np.float64).copy()
+    prm = np.ascontiguousarray(params, np.float64)
+    inc, dt_out = ctypes.c_double(), ctypes.c_double()
+    it = lib.fmso_equilibrium(P(y, _dp), P(prm, _dp), float(dt), int(max_iter), ctypes.byref(inc), ctypes.byref(dt_out))
+    return y, int(it), inc.value, dt_out.value
 
 
 def schedule(lib, *args) -> np.ndarray:
@@ -194,6 +210,41 @@ class OracleBatch:
     def set_only_compute_if_stimulated(self, on: bool) -> None:
         self.prob.only_if_stimulated = int(bool(on))
 
+    def set_algebraics_for_transfer(self, indices) -> None:
+        """algebraicsForTransfer: the last reaction half of every advance() stores
+        these algebraics of its last Heun step's second evaluation"""
+        self._alg_idx = np.ascontiguousarray(indices, np.int32)
+        self._alg = np.zeros((self._alg_idx.size, self.n_total))
+        self.prob.n_alg_transfer = self._alg_idx.size
+        self.prob.alg_transfer_idx = P(self._alg_idx, _ip)
+        self.st.alg_transfer = P(self._alg, _dp)
+
+    def algebraics(self) -> np.ndarray:
+        return self._alg.copy()
+
+    def enable_vc_sweep(self, nmj, width=4, equilibrium_skip=False) -> None:
+        """advance() runs the reference's point-buffer sweep (W nodes per
+        buffer, stimulation and skip decisions per buffer in buffer order)"""
+        n_buf = (self.n_total + width - 1) // width
+        self._vc = dict(nmj=np.ascontiguousarray(np.broadcast_to(nmj, (self.n_fibers,)), np.int32),
+                        eq=np.full(n_buf, 2, np.uint8), cnt=np.zeros(4, np.int64),
+                        pad=np.ascontiguousarray(np.repeat(self.states[:, :1], width, axis=1)))
+        p = self.prob
+        p.vc_width = width
+        p.nmj = P(self._vc["nmj"], _ip)
+        p.equilibrium_skip = int(bool(equilibrium_skip))
+        self.st.vc_eq = P(self._vc["eq"], _ucp)
+        self.st.vc_counters = self._vc["cnt"].ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        self.st.vc_pad = P(self._vc["pad"], _dp)
+
+    def vc_counters(self) -> dict:
+        c = self._vc["cnt"]
+        return dict(inactive_counter=int(c[0]), computed=int(c[1]), skipped_equilibrium=int(c[2]),
+                    skipped_not_stimulated=int(c[3]))
+
+    def vc_equilibrium_states(self) -> np.ndarray:
+        return self._vc["eq"].copy()
+
     def enable_spike_detection(self, threshold=0.0):
         self.prob.spike_threshold = threshold
         self._spk = dict(first=np.full(self.n_total, -1, np.int32), count=np.zeros(self.n_total, np.int32))
@@ -203,6 +254,11 @@ class OracleBatch:
     def advance(self, start_time, dt_split, n_split, dt0D=None, n0D=1, dt1D=None, n1D=1):
         dt0D = 0.5 * dt_split if dt0D is None else dt0D
         dt1D = dt_split if dt1D is None else dt1D
+        if getattr(self, "_vc", None) is not None:
+            rc = self.lib.fmso_advance_vc(ctypes.byref(self.prob), ctypes.byref(self.st), float(start_time),
+                                          float(dt_split), int(n_split), float(dt0D), int(n0D), float(dt1D), int(n1D))
+            assert rc == 0, "fmso_advance_vc: the point-buffer sweep takes firing-table stimulation"
+            return
         self.lib.fmso_advance(ctypes.byref(self.prob), ctypes.byref(self.st), float(start_time), float(dt_split),
                               int(n_split), float(dt0D), int(n0D), float(dt1D), int(n1D), int(self.threads))
 
