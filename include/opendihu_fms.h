@@ -35,6 +35,7 @@ extern "C" {
 #define FMS_ERR_HIP (-2)
 #define FMS_ERR_STATE (-3)
 #define FMS_ERR_UNSUPPORTED (-4)
+#define FMS_ERR_INTERNAL (-5)   /* host memory exhausted or another host-side C++ failure */
 
 /* diffusion time stepping (DiffusionTimeSteppingScheme of FastMonodomainSolverBase) */
 #define FMS_IMPLICIT_EULER 0
@@ -164,6 +165,19 @@ int fms_last_error(char *buf, int len);
                                          fibre (1, default) or one lane per node over the whole batch (0) */
 int fms_set_tuning(FmsSolver *solver, int key, int value);
 
+/* onlyComputeIfHasBeenStimulated in the reference's point-buffer order
+   (fast_monodomain_solver_compute.tpp:131-216): the nodes of the concatenated
+   fibres are swept in buffers of `width` nodes (Vc::double_v::size(), 4 on
+   AVX2), buffer b belonging to fibre floor(b*width/n_nodes[0]); a fibre's
+   stimulation is checked in its centre buffer only (the one holding
+   nmj[f], its stimulation point; a fibre without one is never stimulated),
+   and a buffer swept before the centre buffer in the half step of the first
+   stimulation skips that whole half. Takes effect with
+   FMS_TUNE_ONLY_IF_STIMULATED = 1; runs the split kernels with the reaction per
+   node. width 0 returns to the per-fibre rule (a fibre computes from the Heun
+   step of its first stimulation on). */
+int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [n_fibers] */);
+
 #define FMS_TUNE_PROFILE_KERNELS 7    /* 1: time every launch with HIP events (serialises the host with the GPU) */
 #define FMS_TUNE_PAIR_HALVES 8        /* split kernels: 1 (default) runs the second reaction half of step q and the
                                          first of step q+1 (no solve between them) as one launch that reads and
@@ -196,9 +210,13 @@ int fms_graph_counts(FmsSolver *solver, long long *captures, long long *replays)
    eager or graphs of 8) -
    over the n_steps records of `steps` (4 doubles each, as
    fms_advance_schedule), restores the states, stimulation state and spike
-   counters after every trial and keeps the fastest schedule. candidate_ms
-   (optional, `capacity` entries) receives each candidate's ms per step in the
-   order tried. */
+   counters after every trial and keeps the fastest schedule. Every trial's
+   end state (states, stimulation state machine and log, spike detector) is
+   compared, by a digest of its bits, with that of the solver's own schedule
+   over the same records: a candidate that differs is rejected, never chosen,
+   and counted in n_rejected (if all differ, the solver keeps its schedule and
+   ms_per_step is -1). candidate_ms (optional, `capacity` entries) receives each
+   candidate's ms per step in the order tried, -1 for a rejected one. */
 typedef struct {
   int kernels;            /* FMS_KERNELS_SPLIT / FMS_KERNELS_FUSED */
   int steps_per_launch;
@@ -209,6 +227,7 @@ typedef struct {
   int n_candidates;
   int reaction_per_fibre;
   int reaction_threads;
+  int n_rejected;         /* candidates whose result differed from the solver's own schedule */
 } FmsTuneResult;
 int fms_autotune(FmsSolver *solver, const double *steps, int n_steps, int n0D, int allow_split,
                  FmsTuneResult *result, double *candidate_ms, int capacity);

@@ -39,6 +39,7 @@ class FmsTuneResult(ctypes.Structure):
         ("n_candidates", ctypes.c_int),
         ("reaction_per_fibre", ctypes.c_int),
         ("reaction_threads", ctypes.c_int),
+        ("n_rejected", ctypes.c_int),
     ]
 
 
@@ -109,7 +110,7 @@ EXPORTED_SYMBOLS = [
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
     "fms_eval_rhs", "fms_eval_heun", "fms_adjoint_forward_device", "fms_adjoint_backward_device",
-    "fms_kernel_times",
+    "fms_kernel_times", "fms_set_point_buffer_order",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -162,6 +163,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_schedule.argtypes = [d, d, d, d, d, d, d, d, i, _dp]
     lib.fms_last_error.argtypes = [ctypes.c_char_p, i]
     lib.fms_set_tuning.argtypes = [v, i, i]
+    lib.fms_set_point_buffer_order.argtypes = [v, i, _ip]
     lib.fms_record_event.argtypes = [v, i]
     lib.fms_event_elapsed_ms.argtypes = [v, i, i, _dp]
     lib.fms_launch_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),

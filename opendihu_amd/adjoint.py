@@ -35,6 +35,9 @@ class AdjointProblem:
         self.am_cm = am * cm
         self.initial = batch.get_states()
         self.base_params = batch.parameters.copy()
+        # forwards run so far: the solver keeps the checkpoints, step table and
+        # parameters of the last one only, which the backward reverses
+        self.generation = 0
         lib = batch.lib
         L.check(lib, lib.fms_set_sensors(batch.handle, L.as_ptr(self.sensor_nodes, L._ip), self.n_sensors),
                 "fms_set_sensors")
@@ -49,6 +52,7 @@ class AdjointProblem:
                                                                   L.as_ptr(pf, L._dp)), "fms_set_parameters")
 
     def forward(self, theta) -> np.ndarray:
+        self.generation += 1
         self.set_theta(theta)
         for k in range(self.batch.n_states):
             self.batch.set_state(k, self.initial[k])
@@ -64,6 +68,7 @@ class AdjointProblem:
         torch = _torch()
         if not theta.is_cuda:
             return torch.from_numpy(self.forward(theta.detach().numpy()))
+        self.generation += 1
         self.set_theta(theta.detach().cpu().numpy())
         for k in range(self.batch.n_states):
             self.batch.set_state(k, self.initial[k])
@@ -123,11 +128,21 @@ class MonodomainTraces:
                 @staticmethod
                 def forward(ctx, th, prob):
                     ctx.prob = prob
-                    return prob.forward_tensor(th)
+                    out = prob.forward_tensor(th)
+                    ctx.generation = prob.generation
+                    ctx.theta = th.detach().clone()
+                    return out
 
                 @staticmethod
                 def backward(ctx, g):
-                    return ctx.prob.backward_tensor(g), None
+                    prob = ctx.prob
+                    if prob.generation != ctx.generation:
+                        # another forward ran on the solver since this output
+                        # was made (two outputs in one loss, accumulation,
+                        # gradcheck): recompute this output's forward so the
+                        # reverse sweep uses its own checkpoints and parameters
+                        prob.forward_tensor(ctx.theta)
+                    return prob.backward_tensor(g), None
 
             cls._fn = _Fn
         return cls._fn.apply(theta, problem)

@@ -243,6 +243,7 @@ def generate_hip_model(text: str, name: str, parameters_used_as_algebraic=(), pa
     ops = count_ops(model, p_alg)
     y0 = ", ".join(_cxx_double(v) for v in model.state_initial)
     p0s = ", ".join(_cxx_double(v) for v in p0) if p0 else "0.0"
+    alg_is_param = " || ".join(f"k == {k}" for k in p_alg) if p_alg else "(void)k, false"
     guard = re.sub(r"[^A-Za-z0-9_]", "_", struct).upper()
     src = f"""// Generated by opendihu_amd.cellml from the OpenCOR model "{name}".
 // parametersUsedAsAlgebraic = {p_alg}, parametersUsedAsConstant = {p_const}
@@ -269,6 +270,9 @@ struct {struct} {{
     const double v[NP > 0 ? NP : 1] = {{{p0s}}};
     return v[k];
   }}
+  // algebraics replaced by parameters (parametersUsedAsAlgebraic): not
+  // computed, and not stored as algebraicsForTransfer
+  static FMS_HD constexpr bool alg_is_param(int k) {{ return {alg_is_param}; }}
 
   template <class T, class E>
   static FMS_HD void rhs(const T *y, const T *p, double t, T *r, T *a) {{

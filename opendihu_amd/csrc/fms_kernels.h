@@ -65,6 +65,15 @@ struct DevStim {
   // onlyComputeIfHasBeenStimulated: a fibre's reaction runs from the Heun step
   // of its first stimulation on (fast_monodomain_solver_compute.tpp:201-205)
   int only_if_stimulated;
+  // the reference's point-buffer order (vc_width > 0): nodes are swept in
+  // buffers of vc_width consecutive nodes of the concatenated fibres; buffer b
+  // belongs to fibre floor(b * vc_factor) (vc_factor = width / nodes of fibre 0,
+  // compute.tpp:158-162), a fibre's stimulation is checked in its centre buffer
+  // vc_centre[f] only (-1: none, compute.tpp:163-166), and the has-been flag a
+  // buffer sees is the one set by the buffers swept before it (compute.tpp:172-205)
+  int vc_width;
+  double vc_factor;
+  const int *vc_centre;
 };
 
 // ----------------------------------------------------------------------------
@@ -153,11 +162,15 @@ __device__ __forceinline__ int current_on(const DevStim &s, int f, double t) {
 }
 
 // per Heun step flag byte: bit 0 = stimulate, bit 1 = compute the reaction
-constexpr unsigned char FLAG_STIM = 1, FLAG_COMPUTE = 2;
+// (the fibre has been stimulated by this step), bit 2 = the fibre had been
+// stimulated before the reaction half began (point-buffer order)
+constexpr unsigned char FLAG_STIM = 1, FLAG_COMPUTE = 2, FLAG_HB0 = 4;
 
 __device__ __forceinline__ unsigned char stim_flag(const DevStim &s, int f, double t) {
   int stim = 0;
-  if (s.mode == 1) {
+  if (s.vc_width > 0 && s.vc_centre[f] < 0) {
+    // no centre buffer: the reference never checks this fibre's stimulation
+  } else if (s.mode == 1) {
     stim = stim_check(s, f, t);
   } else if (s.mode == 2) {
     stim = current_on(s, f, t);
@@ -228,7 +241,8 @@ __global__ void k_stim_flags(DevStim st, int n_fibers, unsigned char *flags, Ste
   if (f >= n_fibers) return;
   const double *row = sr.row();
   const double t0 = half_start(row, half), dt = row[2];
-  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, t0 + j * dt);
+  const unsigned char hb0 = st.has_been[f] ? FLAG_HB0 : 0;
+  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, t0 + j * dt) | hb0;
 }
 
 // flags of two consecutive reaction halves, the second half of step q and the
@@ -239,9 +253,29 @@ __global__ void k_stim_flags_pair(DevStim st, int n_fibers, unsigned char *flags
   if (f >= n_fibers) return;
   const double *row = sr.row();
   const double ta = half_start(row, 1), dta = row[2];
-  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, ta + j * dta);
+  const unsigned char hba = st.has_been[f] ? FLAG_HB0 : 0;
+  for (int j = 0; j < n0D; j++) flags[(long long)f * n0D + j] = stim_flag(st, f, ta + j * dta) | hba;
   const double tb = half_start(row + 4, 0), dtb = row[4 + 2];
-  for (int j = 0; j < n0D; j++) flags[((long long)n_fibers + f) * n0D + j] = stim_flag(st, f, tb + j * dtb);
+  const unsigned char hbb = st.has_been[f] ? FLAG_HB0 : 0;
+  for (int j = 0; j < n0D; j++) flags[((long long)n_fibers + f) * n0D + j] = stim_flag(st, f, tb + j * dtb) | hbb;
+}
+
+// Point-buffer order (DevStim::vc_width > 0) with onlyComputeIfHasBeenStimulated:
+// whether node i computes Heun step j of a reaction half. Its buffer b = i / W
+// belongs to fibre fb = floor(b * W / n_0); buffers before fb's centre buffer
+// see the has-been flag of the half's start, the centre buffer the flag after
+// its own check of step j, buffers after it the flag after the whole half.
+// fl points at the half's flags [F][n0D].
+__device__ __forceinline__ bool vc_compute(const DevStim &st, int n_fibers, const unsigned char *fl, int n0D,
+                                           long long i, int j) {
+  const long long b = i / st.vc_width;
+  int fb = (int)((double)b * st.vc_factor);
+  fb = fb < 0 ? 0 : (fb >= n_fibers ? n_fibers - 1 : fb);
+  const unsigned char *ff = fl + (long long)fb * n0D;
+  if (ff[j] & FLAG_HB0) return true;
+  const long long cb = st.vc_centre[fb];
+  if (cb < 0 || b < cb) return false;
+  return (b == cb ? ff[j] : ff[n0D - 1]) & FLAG_COMPUTE;
 }
 
 // ----------------------------------------------------------------------------
@@ -291,8 +325,10 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   for (int j = 0; j < nj; j++) {
     const bool second = PAIR && j >= n0D;
     const int jj = second ? j - n0D : j;
-    const unsigned char fl = flags[(second ? (long long)fb.n_fibers * n0D : 0LL) + (long long)f * n0D + jj];
-    const bool compute = (fl & FLAG_COMPUTE) != 0;
+    const unsigned char *hf = flags + (second ? (long long)fb.n_fibers * n0D : 0LL);
+    const unsigned char fl = hf[(long long)f * n0D + jj];
+    const bool compute = st.vc_width > 0 && st.only_if_stimulated ? vc_compute(st, fb.n_fibers, hf, n0D, i, jj)
+                                                                  : (fl & FLAG_COMPUTE) != 0;
     if (__builtin_amdgcn_ballot_w64(compute) != 0) {   // wave-uniform
       const int on = fl & FLAG_STIM;
       const bool clamp = compute && st.mode == 1 && on && inset;
@@ -320,7 +356,7 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
 #pragma unroll
         for (int q = 0; q < M::NA; q++)
           if (q == a) v = alg[q];
-        alg_transfer[(long long)k * N + i] = v;
+        if (!M::alg_is_param(a)) alg_transfer[(long long)k * N + i] = v;
       }
     }
   }

@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -33,9 +35,40 @@ namespace {
 thread_local std::string g_last_error;
 
 int fail(int code, const std::string &msg) {
-  g_last_error = msg;
+  try {
+    g_last_error = msg;
+  } catch (...) {
+    // no memory for the message: the code still reports the failure
+  }
   return code;
 }
+
+// Every extern "C" entry runs its body inside FMS_API_BEGIN / FMS_API_END: a
+// C++ exception (a host allocation that fails, ...) becomes FMS_ERR_INTERNAL
+// and a message in fms_last_error instead of unwinding across the C ABI.
+int api_exception() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc &) {
+    return fail(FMS_ERR_INTERNAL, "host memory allocation failed");
+  } catch (const std::exception &e) {
+    return fail(FMS_ERR_INTERNAL, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return fail(FMS_ERR_INTERNAL, "internal error: unknown C++ exception");
+  }
+}
+
+#define FMS_API_BEGIN try {
+#define FMS_API_END                                                                                \
+  }                                                                                                \
+  catch (...) {                                                                                    \
+    return api_exception();                                                                        \
+  }
+#define FMS_API_END_VOID                                                                           \
+  }                                                                                                \
+  catch (...) {                                                                                    \
+    (void)api_exception();                                                                         \
+  }
 
 #define HIP_TRY(expr)                                                                              \
   do {                                                                                             \
@@ -101,6 +134,10 @@ struct FmsSolver {
   int stim_param = 0;
   int only_if_stimulated = 0;
   int fe_rows = 1, fe_cols = 1;
+  // the reference's point-buffer order (fms_set_point_buffer_order)
+  int vc_width = 0;
+  double vc_factor = 0.0;
+  int *d_vc_centre = nullptr;
 
   // spikes
   int spikes = 0;
@@ -196,6 +233,9 @@ struct FmsSolver {
     s.amplitude = d_amp;
     s.stim_param = stim_param;
     s.only_if_stimulated = only_if_stimulated;
+    s.vc_width = vc_width;
+    s.vc_factor = vc_factor;
+    s.vc_centre = d_vc_centre;
     return s;
   }
 };
@@ -209,7 +249,7 @@ void free_all(FmsSolver *s) {
                   s->d_curjit, s->d_curr, s->d_hasbeen, s->d_log_count, s->d_log_fiber, s->d_log_time,
                   s->d_onset, s->d_dur, s->d_amp, s->d_vprev, s->d_sfirst, s->d_scount, s->d_alg_idx,
                   s->d_alg, s->d_sensor_nodes, s->d_sensor_slot, s->d_trace, s->d_dtrace, s->d_ckpt, s->d_tape,
-                  s->d_lam, s->d_gacc, s->d_pfbar, s->d_grad, s->d_cursor};
+                  s->d_lam, s->d_gacc, s->d_pfbar, s->d_grad, s->d_cursor, s->d_vc_centre};
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -309,7 +349,7 @@ int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
   const fms::DevSpikes off{};
-  if (Model::NS > 8 && !store_alg && s->tune_reaction_fibre) {
+  if (Model::NS > 8 && !store_alg && s->tune_reaction_fibre && !s->vc_width) {
     launch_reaction_fibre<G, E, false>(s, q, half, n0D, off);
   } else if (store_alg)
     hipLaunchKernelGGL((fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
@@ -330,7 +370,7 @@ template <class G, class E>
 int launch_reaction_pair(FmsSolver *s, int q, int n0D) {
   KernelTimer timer(s, FMS_KERNEL_REACTION_PAIR);
   const fms::DevSpikes spk = dev_spikes(s);
-  if (Model::NS > 8 && s->tune_reaction_fibre) {
+  if (Model::NS > 8 && s->tune_reaction_fibre && !s->vc_width) {
     launch_reaction_fibre<G, E, true>(s, q, 1, n0D, spk);
   } else {
     const int threads = 256;
@@ -573,7 +613,13 @@ int run(FmsSolver *s, const double *h_steps, int K, int n0D, bool fused_only);
 
 template <class G, class E>
 int run(FmsSolver *s, const double *h_steps, int K, int n0D) {
-  if (s->kernels == FMS_KERNELS_SPLIT) return run_split<G, E>(s, h_steps, K, n0D);
+  // the point-buffer order decides per node: the per-node split kernels
+  if (s->kernels == FMS_KERNELS_SPLIT || s->vc_width) {
+    if (s->thomas == FMS_THOMAS_SERIAL && !s->d_cA &&
+        (dalloc(&s->d_cA, (size_t)s->N) || dalloc(&s->d_dA, (size_t)s->N)))
+      return FMS_ERR_HIP;
+    return run_split<G, E>(s, h_steps, K, n0D);
+  }
   // the fused kernel does not return algebraics: with algebraicsForTransfer
   // requested, the last step of the call runs on the split kernels (same
   // arithmetic, bit-identical states)
@@ -626,36 +672,45 @@ size_t gate_bytes(const FmsSolver *s) { return s->precision == FMS_FP64 ? sizeof
 extern "C" {
 
 int fms_last_error(char *buf, int len) {
+  FMS_API_BEGIN
   if (!buf || len <= 0) return FMS_ERR_ARGUMENT;
   snprintf(buf, (size_t)len, "%s", g_last_error.c_str());
   return 0;
+  FMS_API_END
 }
 
 int fms_model_info(int *n_states, int *n_algebraics, int *n_params, char *name, int name_len) {
+  FMS_API_BEGIN
   if (n_states) *n_states = Model::NS;
   if (n_algebraics) *n_algebraics = Model::NA;
   if (n_params) *n_params = Model::NP;
   if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", Model::name());
   return 0;
+  FMS_API_END
 }
 
 int fms_model_ops(double *ops_rhs, double *divs_rhs, double *exps_rhs) {
+  FMS_API_BEGIN
   if (ops_rhs) *ops_rhs = Model::OPS_RHS;
   if (divs_rhs) *divs_rhs = Model::DIVS_RHS;
   if (exps_rhs) *exps_rhs = Model::EXPS_RHS;
   return 0;
+  FMS_API_END
 }
 
 int fms_model_defaults(double *states, double *params) {
+  FMS_API_BEGIN
   if (states)
     for (int k = 0; k < Model::NS; k++) states[k] = Model::y0(k);
   if (params)
     for (int k = 0; k < Model::NP; k++) params[k] = Model::p0(k);
   return 0;
+  FMS_API_END
 }
 
 int fms_eval_rhs(const double *states, const double *params, double t, int n, int exp_mode, double *rates,
                  double *algebraics) {
+  FMS_API_BEGIN
   if (n <= 0 || !states || !rates || (Model::NP > 0 && !params)) return fail(FMS_ERR_ARGUMENT, "bad arguments");
   if (exp_mode != FMS_EXP_APPROXIMATE && exp_mode != FMS_EXP_LIBM) return fail(FMS_ERR_ARGUMENT, "bad exp_mode");
   const size_t NS = Model::NS, NA = Model::NA, NP = Model::NP > 0 ? Model::NP : 1;
@@ -682,9 +737,11 @@ int fms_eval_rhs(const double *states, const double *params, double t, int n, in
   for (double *q : {d_y, d_p, d_r, d_a})
     if (q) (void)hipFree(q);
   return rc ? fail(rc, "fms_eval_rhs: device error") : 0;
+  FMS_API_END
 }
 
 int fms_eval_heun(double *states, const double *params, double t0, double dt, int n_steps, int n, int exp_mode) {
+  FMS_API_BEGIN
   if (n <= 0 || n_steps < 0 || !states || (Model::NP > 0 && !params)) return fail(FMS_ERR_ARGUMENT, "bad arguments");
   if (exp_mode != FMS_EXP_APPROXIMATE && exp_mode != FMS_EXP_LIBM) return fail(FMS_ERR_ARGUMENT, "bad exp_mode");
   const size_t NS = Model::NS, NP = Model::NP > 0 ? Model::NP : 1;
@@ -708,9 +765,11 @@ int fms_eval_heun(double *states, const double *params, double t0, double dt, in
   for (double *q : {d_y, d_p})
     if (q) (void)hipFree(q);
   return rc ? fail(rc, "fms_eval_heun: device error") : 0;
+  FMS_API_END
 }
 
 int fms_create(const FmsConfig *c, FmsSolver **out) {
+  FMS_API_BEGIN
   if (!c || !out) return fail(FMS_ERR_ARGUMENT, "null argument");
   *out = nullptr;
   if (c->n_fibers <= 0 || !c->n_nodes || !c->element_lengths || !c->prefactor)
@@ -835,17 +894,21 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
   }
   *out = s;
   return 0;
+  FMS_API_END
 }
 
 int fms_destroy(FmsSolver *s) {
+  FMS_API_BEGIN
   if (!s) return 0;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_all(s);
   delete s;
   return 0;
+  FMS_API_END
 }
 
 int fms_set_stimulation(FmsSolver *s, const FmsStimulation *st) {
+  FMS_API_BEGIN
   if (!s || !st) return fail(FMS_ERR_ARGUMENT, "null argument");
   const int F = s->F;
   if (st->mode == FMS_STIM_NONE) {
@@ -927,9 +990,11 @@ int fms_set_stimulation(FmsSolver *s, const FmsStimulation *st) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->stim_mode = st->mode;
   return 0;
+  FMS_API_END
 }
 
 int fms_advance(FmsSolver *s, double t0, double dts, int n_split, double dt0D, int n0D, double dt1D, int n1D) {
+  FMS_API_BEGIN
   (void)n1D;
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   if (n_split <= 0) return 0;
@@ -941,26 +1006,34 @@ int fms_advance(FmsSolver *s, double t0, double dts, int n_split, double dt0D, i
     tab[4 * (size_t)k + 3] = dt1D;
   }
   return dispatch(s, tab.data(), n_split, n0D);
+  FMS_API_END
 }
 
 int fms_advance_schedule(FmsSolver *s, const double *steps, int n_steps, int n0D) {
+  FMS_API_BEGIN
   if (!s || (!steps && n_steps > 0)) return fail(FMS_ERR_ARGUMENT, "null argument");
   return dispatch(s, steps, n_steps, n0D);
+  FMS_API_END
 }
 
 int fms_synchronize(FmsSolver *s) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   HIP_TRY(hipStreamSynchronize(s->stream));
   return 0;
+  FMS_API_END
 }
 
 int fms_total_nodes(FmsSolver *s, long long *n) {
+  FMS_API_BEGIN
   if (!s || !n) return fail(FMS_ERR_ARGUMENT, "null argument");
   *n = s->N;
   return 0;
+  FMS_API_END
 }
 
 int fms_get_states(FmsSolver *s, int k, double *out) {
+  FMS_API_BEGIN
   if (!s || !out) return fail(FMS_ERR_ARGUMENT, "null argument");
   if (k < 0 || k >= Model::NS) return fail(FMS_ERR_ARGUMENT, "state index out of range");
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -976,9 +1049,11 @@ int fms_get_states(FmsSolver *s, int k, double *out) {
     for (long long i = 0; i < s->N; i++) out[i] = tmp[(size_t)i];
   }
   return 0;
+  FMS_API_END
 }
 
 int fms_set_states(FmsSolver *s, int k, const double *in) {
+  FMS_API_BEGIN
   if (!s || !in) return fail(FMS_ERR_ARGUMENT, "null argument");
   if (k < 0 || k >= Model::NS) return fail(FMS_ERR_ARGUMENT, "state index out of range");
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -994,9 +1069,11 @@ int fms_set_states(FmsSolver *s, int k, const double *in) {
                       hipMemcpyHostToDevice));
   }
   return 0;
+  FMS_API_END
 }
 
 int fms_set_algebraics_for_transfer(FmsSolver *s, const int *idx, int count) {
+  FMS_API_BEGIN
   if (!s || count < 0 || (count > 0 && !idx)) return fail(FMS_ERR_ARGUMENT, "bad argument");
   for (int k = 0; k < count; k++)
     if (idx[k] < 0 || idx[k] >= Model::NA) return fail(FMS_ERR_ARGUMENT, "algebraic index out of range");
@@ -1011,17 +1088,21 @@ int fms_set_algebraics_for_transfer(FmsSolver *s, const int *idx, int count) {
   HIP_TRY(hipMemcpy(s->d_alg_idx, idx, count * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(s->d_alg, 0, (size_t)count * s->N * sizeof(double)));
   return 0;
+  FMS_API_END
 }
 
 int fms_get_algebraics(FmsSolver *s, double *out) {
+  FMS_API_BEGIN
   if (!s || !out) return fail(FMS_ERR_ARGUMENT, "null argument");
   if (!s->d_alg) return fail(FMS_ERR_STATE, "no algebraics for transfer configured");
   HIP_TRY(hipStreamSynchronize(s->stream));
   HIP_TRY(hipMemcpy(out, s->d_alg, (size_t)s->n_alg_idx * s->N * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
+  FMS_API_END
 }
 
 int fms_enable_spike_detection(FmsSolver *s, double th) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (!s->d_vprev) {
@@ -1035,18 +1116,22 @@ int fms_enable_spike_detection(FmsSolver *s, double th) {
   s->spikes = 1;
   s->spike_th = th;
   return 0;
+  FMS_API_END
 }
 
 int fms_get_spikes(FmsSolver *s, int *first, int *count) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   if (!s->spikes) return fail(FMS_ERR_STATE, "spike detection is not enabled");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (first) HIP_TRY(hipMemcpy(first, s->d_sfirst, (size_t)s->N * sizeof(int), hipMemcpyDeviceToHost));
   if (count) HIP_TRY(hipMemcpy(count, s->d_scount, (size_t)s->N * sizeof(int), hipMemcpyDeviceToHost));
   return 0;
+  FMS_API_END
 }
 
 int fms_get_stimulation_log(FmsSolver *s, int *count, double *times, int *fibers, int cap) {
+  FMS_API_BEGIN
   if (!s || !count) return fail(FMS_ERR_ARGUMENT, "null argument");
   *count = 0;
   if (s->stim_mode != FMS_STIM_FIRING_TABLE || s->log_cap <= 0) return 0;
@@ -1056,9 +1141,11 @@ int fms_get_stimulation_log(FmsSolver *s, int *count, double *times, int *fibers
   if (n > 0 && times) HIP_TRY(hipMemcpy(times, s->d_log_time, n * sizeof(double), hipMemcpyDeviceToHost));
   if (n > 0 && fibers) HIP_TRY(hipMemcpy(fibers, s->d_log_fiber, n * sizeof(int), hipMemcpyDeviceToHost));
   return 0;
+  FMS_API_END
 }
 
 int fms_set_tuning(FmsSolver *s, int key, int value) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   switch (key) {
     case FMS_TUNE_STEPS_PER_LAUNCH:
@@ -1112,16 +1199,54 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
     default:
       return fail(FMS_ERR_ARGUMENT, "unknown tuning key");
   }
+  FMS_API_END
+}
+
+int fms_set_point_buffer_order(FmsSolver *s, int width, const int *nmj) {
+  FMS_API_BEGIN
+  if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  if (width < 0 || width > 64) return fail(FMS_ERR_ARGUMENT, "point-buffer width must be 0..64");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (width == 0) {
+    s->vc_width = 0;
+    return 0;
+  }
+  if (!nmj) return fail(FMS_ERR_ARGUMENT, "the point-buffer order needs each fibre's stimulation point");
+  // centre buffer of each fibre as the sweep finds it (compute.tpp:158-166):
+  // buffer b belongs to fibre floor(b * W / n_0) and is its centre if
+  // (unsigned long)(nmj - (b*W - offset)) < W
+  const int F = s->F;
+  const long long nbuf = (s->N + width - 1) / width;
+  const double factor = (double)width / (s->h_off[1] - s->h_off[0]);
+  std::vector<int> centre(F, -1);
+  for (long long b = 0; b < nbuf; b++) {
+    int fb = (int)((double)b * factor);
+    fb = std::min(std::max(fb, 0), F - 1);
+    const long long index_in_fibre = b * width - s->h_off[fb];
+    if ((unsigned long)((long long)nmj[fb] - index_in_fibre) < (unsigned long)width) {
+      if (b > 0x7fffffffLL) return fail(FMS_ERR_UNSUPPORTED, "too many point buffers");
+      centre[fb] = (int)b;
+    }
+  }
+  if (!s->d_vc_centre && dalloc(&s->d_vc_centre, (size_t)F)) return FMS_ERR_HIP;
+  HIP_TRY(hipMemcpy(s->d_vc_centre, centre.data(), F * sizeof(int), hipMemcpyHostToDevice));
+  s->vc_width = width;
+  s->vc_factor = factor;
+  return 0;
+  FMS_API_END
 }
 
 int fms_record_event(FmsSolver *s, int slot) {
+  FMS_API_BEGIN
   if (!s || slot < 0 || slot >= 8) return fail(FMS_ERR_ARGUMENT, "event slot must be 0..7");
   if (!s->events[slot]) HIP_TRY(hipEventCreate(&s->events[slot]));
   HIP_TRY(hipEventRecord(s->events[slot], s->stream));
   return 0;
+  FMS_API_END
 }
 
 int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
+  FMS_API_BEGIN
   if (!s || !ms || from < 0 || from >= 8 || to < 0 || to >= 8 || !s->events[from] || !s->events[to])
     return fail(FMS_ERR_ARGUMENT, "events not recorded");
   HIP_TRY(hipEventSynchronize(s->events[to]));
@@ -1129,9 +1254,11 @@ int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
   HIP_TRY(hipEventElapsedTime(&f, s->events[from], s->events[to]));
   *ms = f;
   return 0;
+  FMS_API_END
 }
 
 int fms_kernel_times(FmsSolver *s, double *ms, long long *launches, int reset) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   for (int k = 0; k < FMS_KERNEL_CLASSES; k++) {
     if (ms) ms[k] = s->prof_ms[k];
@@ -1142,21 +1269,26 @@ int fms_kernel_times(FmsSolver *s, double *ms, long long *launches, int reset) {
     }
   }
   return 0;
+  FMS_API_END
 }
 
 int fms_graph_counts(FmsSolver *s, long long *captures, long long *replays) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   if (captures) *captures = s->n_graph_captures;
   if (replays) *replays = s->n_graph_launches;
   return 0;
+  FMS_API_END
 }
 
 int fms_launch_counts(FmsSolver *s, long long *fused, long long *reaction, long long *thomas) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   if (fused) *fused = s->n_launch_fused;
   if (reaction) *reaction = s->n_launch_reaction;
   if (thomas) *thomas = s->n_launch_thomas;
   return 0;
+  FMS_API_END
 }
 
 }  // extern "C"
@@ -1322,15 +1454,18 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
 extern "C" {
 
 int fms_set_parameters(FmsSolver *s, const double *params, const double *prefactor) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (params && Model::NP > 0)
     HIP_TRY(hipMemcpy(s->d_params, params, (size_t)s->F * Model::NP * sizeof(double), hipMemcpyHostToDevice));
   if (prefactor) HIP_TRY(hipMemcpy(s->d_pf, prefactor, (size_t)s->F * sizeof(double), hipMemcpyHostToDevice));
   return 0;
+  FMS_API_END
 }
 
 int fms_set_sensors(FmsSolver *s, const int *sensor_nodes, int n_sensors) {
+  FMS_API_BEGIN
   if (!s || n_sensors < 0 || (n_sensors > 0 && !sensor_nodes)) return fail(FMS_ERR_ARGUMENT, "bad sensors");
   const int F = s->F;
   std::vector<int> slot((size_t)s->N, -1);
@@ -1354,10 +1489,12 @@ int fms_set_sensors(FmsSolver *s, const int *sensor_nodes, int n_sensors) {
   HIP_TRY(hipMemcpy(s->d_sensor_nodes, sensor_nodes, (size_t)F * n_sensors * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d_sensor_slot, slot.data(), (size_t)s->N * sizeof(int), hipMemcpyHostToDevice));
   return 0;
+  FMS_API_END
 }
 
 int fms_adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int checkpoint_interval,
                         double *traces) {
+  FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
   int rc;
   if (n_steps > 0 && s->n_sensors > 0 &&
@@ -1370,18 +1507,22 @@ int fms_adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D,
                       hipMemcpyDeviceToHost));
   }
   return 0;
+  FMS_API_END
 }
 
 int fms_adjoint_forward_device(FmsSolver *s, const double *steps, int n_steps, int n0D, int checkpoint_interval,
                                double *d_traces) {
+  FMS_API_BEGIN
   if (!s || !d_traces) return fail(FMS_ERR_ARGUMENT, "null argument");
   int rc;
   if ((rc = adjoint_forward(s, steps, n_steps, n0D, checkpoint_interval, d_traces))) return rc;
   HIP_TRY(hipStreamSynchronize(s->stream));
   return 0;
+  FMS_API_END
 }
 
 int fms_adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_params, double *grad_prefactor) {
+  FMS_API_BEGIN
   if (!s || !dtraces) return fail(FMS_ERR_ARGUMENT, "null argument");
   if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
   const size_t len = (size_t)s->adj_K * s->F * s->n_sensors;
@@ -1395,10 +1536,12 @@ int fms_adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_param
   if (grad_prefactor)
     HIP_TRY(hipMemcpy(grad_prefactor, s->d_pfbar, (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
+  FMS_API_END
 }
 
 int fms_adjoint_backward_device(FmsSolver *s, const double *d_dtraces, double *d_grad_params,
                                 double *d_grad_prefactor) {
+  FMS_API_BEGIN
   if (!s || !d_dtraces) return fail(FMS_ERR_ARGUMENT, "null argument");
   if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
   int rc;
@@ -1411,6 +1554,7 @@ int fms_adjoint_backward_device(FmsSolver *s, const double *d_dtraces, double *d
                            s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   return 0;
+  FMS_API_END
 }
 
 }  // extern "C"
@@ -1481,6 +1625,68 @@ void snapshot_free(Snapshot &sn) {
   sn.copies.clear();
 }
 
+// Digest of the device state a trial leaves behind, to check that a candidate
+// schedule computed the same bits as the solver's own schedule. Each region's
+// 8-byte words are mixed with their position and summed (mod 2^64), so the
+// reduction order does not matter; the stimulation log, whose entries land in
+// atomic order, enters as an order-free sum over its (time, fibre) pairs.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void digest_add(unsigned long long acc, unsigned long long *out) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+__global__ void k_digest(const unsigned char *p, unsigned long long bytes, unsigned long long salt,
+                         unsigned long long *out) {
+  const unsigned long long nw = bytes / 8;
+  const unsigned long long *w = reinterpret_cast<const unsigned long long *>(p);
+  unsigned long long acc = 0;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    acc += mix64(w[i] ^ mix64(i + salt));
+  if (blockIdx.x == 0 && threadIdx.x == 0 && bytes % 8) {
+    unsigned long long t = 0;
+    for (unsigned long long k = nw * 8; k < bytes; k++) t = (t << 8) | p[k];
+    acc += mix64(t ^ mix64(nw + salt));
+  }
+  digest_add(acc, out);
+}
+
+__global__ void k_digest_log(const double *times, const int *fibers, const int *count, int cap,
+                             unsigned long long *out) {
+  const int n = min(*count, cap);
+  unsigned long long acc = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    acc += mix64((unsigned long long)__double_as_longlong(times[k]) ^ mix64((unsigned long long)(unsigned)fibers[k]));
+  digest_add(acc, out);
+}
+
+int state_digest(FmsSolver *s, const Snapshot &sn, unsigned long long *d_out, unsigned long long &h) {
+  HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(unsigned long long), s->stream));
+  for (size_t k = 0; k < sn.regions.size(); k++) {
+    const unsigned long long bytes = sn.regions[k].second;
+    const unsigned blocks = (unsigned)std::min<unsigned long long>(1024, (bytes / 8 + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_digest, dim3(blocks), dim3(256), 0, s->stream, (const unsigned char *)sn.regions[k].first,
+                       bytes, (unsigned long long)(k + 1) << 40, d_out);
+    HIP_TRY(hipGetLastError());
+  }
+  if (s->stim_mode == FMS_STIM_FIRING_TABLE && s->log_cap > 0 && s->d_log_count) {
+    hipLaunchKernelGGL(k_digest_log, dim3(64), dim3(256), 0, s->stream, s->d_log_time, s->d_log_fiber,
+                       s->d_log_count, s->log_cap, d_out);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(&h, d_out, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
 struct Schedule {
   int kernels, chunk, resident, threads, graph, per_fibre, rthreads;
 };
@@ -1541,6 +1747,7 @@ extern "C" {
 
 int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int allow_split, FmsTuneResult *result,
                  double *candidate_ms, int capacity) {
+  FMS_API_BEGIN
   if (!s || !steps || n_steps < 1 || n0D < 1) return fail(FMS_ERR_ARGUMENT, "bad autotune arguments");
   if (s->d_alg) return fail(FMS_ERR_STATE, "autotune before requesting algebraics for transfer");
   const std::vector<Schedule> cand = candidates(s, allow_split);
@@ -1552,8 +1759,14 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(FMS_ERR_HIP, "hipEventCreate failed");
-  int best = -1;
+  int best = -1, rejected = 0;
   double best_ms = 0;
+  // the solver's own schedule computes the reference digest
+  unsigned long long *d_digest = nullptr, ref_digest = 0;
+  if (!rc) rc = dalloc(&d_digest, 1);
+  if (!rc) rc = snapshot_restore(s, sn);
+  if (!rc) rc = dispatch(s, steps, n_steps, n0D);
+  if (!rc) rc = state_digest(s, sn, d_digest, ref_digest);
   for (size_t k = 0; k < cand.size() && !rc; k++) {
     if ((rc = apply_schedule(s, cand[k]))) break;
     // one untimed pass (graph capture, code object load), then the timed pass
@@ -1570,12 +1783,23 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
     }
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    if (candidate_ms && (int)k < capacity) candidate_ms[k] = ms / n_steps;
+    // a candidate whose states, stimulation state machine, log or spike data
+    // differ from the solver's own schedule's is reported (negative time) and
+    // never chosen
+    unsigned long long dig = 0;
+    if ((rc = state_digest(s, sn, d_digest, dig))) break;
+    const bool same = dig == ref_digest;
+    if (candidate_ms && (int)k < capacity) candidate_ms[k] = same ? ms / n_steps : -1.0;
+    if (!same) {
+      rejected++;
+      continue;
+    }
     if (best < 0 || ms < best_ms) {
       best = (int)k;
       best_ms = ms;
     }
   }
+  if (d_digest) (void)hipFree(d_digest);
   if (!rc) rc = snapshot_restore(s, sn);
   if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FMS_ERR_HIP, "hipStreamSynchronize");
   snapshot_free(sn);
@@ -1585,19 +1809,23 @@ int fms_autotune(FmsSolver *s, const double *steps, int n_steps, int n0D, int al
     (void)apply_schedule(s, keep);
     return rc;
   }
-  if ((rc = apply_schedule(s, cand[best]))) return rc;
+  // every candidate rejected: the solver keeps its own schedule
+  const Schedule chosen = best >= 0 ? cand[best] : keep;
+  if ((rc = apply_schedule(s, chosen))) return rc;
   if (result) {
-    result->kernels = cand[best].kernels;
-    result->steps_per_launch = cand[best].chunk;
-    result->resident_gates = cand[best].resident;
-    result->threads = cand[best].threads;
-    result->graph_steps = cand[best].graph;
-    result->reaction_per_fibre = cand[best].per_fibre;
-    result->reaction_threads = cand[best].rthreads;
-    result->ms_per_step = best_ms / n_steps;
+    result->kernels = chosen.kernels;
+    result->steps_per_launch = chosen.chunk;
+    result->resident_gates = chosen.resident;
+    result->threads = chosen.threads;
+    result->graph_steps = chosen.graph;
+    result->reaction_per_fibre = chosen.per_fibre;
+    result->reaction_threads = chosen.rthreads;
+    result->ms_per_step = best >= 0 ? best_ms / n_steps : -1.0;
     result->n_candidates = (int)cand.size();
+    result->n_rejected = rejected;
   }
   return 0;
+  FMS_API_END
 }
 
 }  // extern "C"

@@ -13,9 +13,13 @@
 //   * Vm and the gates live in the library between calls; initializeArrays
 //     sets every instance to statesOneInstance (gpu.tpp:407-467).
 //   * The stimulated nodes are NMJ-1..NMJ+1 (gpu.tpp:540).
-//   * parameters[p*nInstances + i] are per instance; the fibre solver takes
-//     the first instance of each fibre (the reference host fills every
-//     instance with the same value, initialization.tpp:459-465).
+//   * parameters[p*nInstances + i] are per instance; the fibre solver holds one
+//     set per fibre (the reference host fills every instance of a fibre with
+//     the same value, initialization.tpp:459-465). Parameters and prefactor are
+//     taken on every call (gpu.tpp:481,490) and uploaded when they change; a
+//     call whose instances of one fibre differ fails (FMS_ERR_UNSUPPORTED).
+//   * A void entry clears fms_last_error first: an empty message after the
+//     call means success.
 //   * After the call, statesForTransfer[i*nInstances + inst] holds state
 //     statesForTransferIndices[i] (index 0 is Vm) and algebraicsForTransfer the
 //     requested algebraics of the last Heun step (gpu.tpp:401-461).
@@ -41,6 +45,8 @@ struct RefAbi {
   std::vector<int> jit_off, jit_cnt, mu, nmj;
   std::vector<double> last_check, freq, repeat, begin;
   std::vector<float> last_lengths;
+  std::vector<double> last_params;
+  double last_prefactor = 0.0;
 };
 
 RefAbi g_ref;
@@ -80,9 +86,37 @@ bool ref_dims_from_env() {
   return true;
 }
 
+// parameters[p*N + inst] (float, N = F*n instances) as one set per fibre
+// [F][NP]; parameters past n_params_given keep the model's defaults. The fibre
+// solver holds one parameter set per fibre, so a call whose instances of one
+// fibre differ is refused (FMS_ERR_UNSUPPORTED) rather than computed with the
+// first instance's values.
+bool ref_fibre_parameters(const float *parameters, int F, int n, int n_params_given, std::vector<double> &prm) {
+  const long long N = (long long)F * n;
+  prm.assign((size_t)F * (Model::NP > 0 ? Model::NP : 1), 0.0);
+  for (int f = 0; f < F; f++)
+    for (int k = 0; k < Model::NP; k++) {
+      if (k >= n_params_given || !parameters) {
+        prm[(size_t)f * Model::NP + k] = Model::p0(k);
+        continue;
+      }
+      const float *row = parameters + (size_t)k * N + (size_t)f * n;
+      for (int i = 1; i < n; i++)
+        if (!(row[i] == row[0])) {
+          fail(FMS_ERR_UNSUPPORTED, "computeMonodomain: parameter " + std::to_string(k) + " differs between nodes of fibre " +
+                                        std::to_string(f) + " (node " + std::to_string(i) +
+                                        "); the fibre solver takes one parameter set per fibre");
+          return false;
+        }
+      prm[(size_t)f * Model::NP + k] = (double)row[0];
+    }
+  return true;
+}
+
 }  // namespace
 
 extern "C" int fms_reference_set_dims(const FmsReferenceDims *d) {
+  FMS_API_BEGIN
   if (!d) return fail(FMS_ERR_ARGUMENT, "null dims");
   if (d->n_fibers_to_compute <= 0 || d->n_instances_per_fiber < 2 || d->firing_events_rows <= 0 ||
       d->firing_events_cols <= 0 || d->n_states_for_transfer < 1 || d->n_parameters_per_instance < 0 ||
@@ -96,6 +130,7 @@ extern "C" int fms_reference_set_dims(const FmsReferenceDims *d) {
   g_ref.dims = *d;
   g_ref.dims_set = true;
   return 0;
+  FMS_API_END
 }
 
 extern "C" void initializeArrays(const double *statesOneInstance, const int *algebraicsForTransferIndices,
@@ -105,6 +140,8 @@ extern "C" void initializeArrays(const double *statesOneInstance, const int *alg
                                  const double *setSpecificStatesCallFrequency,
                                  const double *setSpecificStatesRepeatAfterFirstCall,
                                  const double *setSpecificStatesCallEnableBegin) {
+  FMS_API_BEGIN
+  g_last_error.clear();
   if (!g_ref.dims_set && !ref_dims_from_env()) {
     fail(FMS_ERR_STATE, "initializeArrays: sizes unknown (call fms_reference_set_dims or set OPENDIHU_FMS_DIMS)");
     return;
@@ -133,12 +170,15 @@ extern "C" void initializeArrays(const double *statesOneInstance, const int *alg
   g_ref.repeat.assign(setSpecificStatesRepeatAfterFirstCall, setSpecificStatesRepeatAfterFirstCall + F);
   g_ref.begin.assign(setSpecificStatesCallEnableBegin, setSpecificStatesCallEnableBegin + F);
   g_ref.initialized = true;
+  FMS_API_END_VOID
 }
 
 extern "C" void computeMonodomain(const float *parameters, double *algebraicsForTransfer, double *statesForTransfer,
                                   const float *elementLengths, double startTime, double timeStepWidthSplitting,
                                   int nTimeStepsSplitting, double dt0D, int nTimeSteps0D, double dt1D,
                                   int nTimeSteps1D, double prefactor, double valueForStimulatedPoint) {
+  FMS_API_BEGIN
+  g_last_error.clear();
   if (!g_ref.initialized) {
     fail(FMS_ERR_STATE, "computeMonodomain called before initializeArrays");
     return;
@@ -147,15 +187,14 @@ extern "C" void computeMonodomain(const float *parameters, double *algebraicsFor
   const int F = d.n_fibers_to_compute, n = d.n_instances_per_fiber;
   const long long N = (long long)F * n;
   const size_t nlen = (size_t)F * (n - 1);
+  // parameters and prefactor of this call (the reference passes both on every
+  // call, gpu.tpp:481,490): one parameter set per fibre
+  std::vector<double> prm;
+  if (!ref_fibre_parameters(parameters, F, n, d.n_parameters_per_instance, prm)) return;
   if (!g_ref.solver) {
     std::vector<int> nn(F, n);
     std::vector<double> h(nlen), pf(F, prefactor);
     for (size_t i = 0; i < nlen; i++) h[i] = elementLengths[i];
-    std::vector<double> prm((size_t)F * (Model::NP > 0 ? Model::NP : 1));
-    for (int f = 0; f < F; f++)
-      for (int k = 0; k < Model::NP; k++)
-        prm[(size_t)f * Model::NP + k] =
-            k < d.n_parameters_per_instance ? (double)parameters[(size_t)k * N + (size_t)f * n] : Model::p0(k);
     FmsConfig c{};
     c.n_fibers = F;
     c.n_nodes = nn.data();
@@ -195,21 +234,35 @@ extern "C" void computeMonodomain(const float *parameters, double *algebraicsFor
         fms_set_algebraics_for_transfer(g_ref.solver, g_ref.alg_idx.data(), (int)g_ref.alg_idx.size()))
       return;
     g_ref.last_lengths.assign(elementLengths, elementLengths + nlen);
-  } else if (memcmp(g_ref.last_lengths.data(), elementLengths, nlen * sizeof(float)) != 0) {
-    // geometry moved (e.g. a coupled contraction solver): upload the new lengths
-    std::vector<double> h(nlen);
-    for (size_t i = 0; i < nlen; i++) h[i] = elementLengths[i];
-    if (hipMemcpy(g_ref.solver->d_h, h.data(), nlen * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-      fail(FMS_ERR_HIP, "element length upload failed");
-      return;
+    g_ref.last_params = prm;
+    g_ref.last_prefactor = prefactor;
+  } else {
+    if (memcmp(g_ref.last_lengths.data(), elementLengths, nlen * sizeof(float)) != 0) {
+      // geometry moved (e.g. a coupled contraction solver): upload the new lengths
+      std::vector<double> h(nlen);
+      for (size_t i = 0; i < nlen; i++) h[i] = elementLengths[i];
+      if (hipStreamSynchronize(g_ref.solver->stream) != hipSuccess ||
+          hipMemcpy(g_ref.solver->d_h, h.data(), nlen * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        fail(FMS_ERR_HIP, "element length upload failed");
+        return;
+      }
+      g_ref.last_lengths.assign(elementLengths, elementLengths + nlen);
     }
-    g_ref.last_lengths.assign(elementLengths, elementLengths + nlen);
+    // parameters (e.g. parametersUsedAsAlgebraic fed back by a coupled solver)
+    // and the prefactor: uploaded when they differ from the previous call's
+    const bool new_params = prm != g_ref.last_params;
+    const bool new_pf = !(prefactor == g_ref.last_prefactor);
+    if (new_params || new_pf) {
+      std::vector<double> pf(F, prefactor);
+      if (fms_set_parameters(g_ref.solver, new_params ? prm.data() : nullptr, new_pf ? pf.data() : nullptr)) return;
+      g_ref.last_params = prm;
+      g_ref.last_prefactor = prefactor;
+    }
   }
   g_ref.solver->vstim = valueForStimulatedPoint;
   if (fms_advance(g_ref.solver, startTime, timeStepWidthSplitting, nTimeStepsSplitting, dt0D, nTimeSteps0D, dt1D,
                   nTimeSteps1D))
     return;
-  std::vector<double> tmp((size_t)N);
   for (size_t i = 0; i < g_ref.state_idx.size(); i++) {
     const int k = g_ref.state_idx[i];
     if (k < 0 || k >= Model::NS) {
@@ -219,4 +272,5 @@ extern "C" void computeMonodomain(const float *parameters, double *algebraicsFor
     if (fms_get_states(g_ref.solver, k, statesForTransfer + i * (size_t)N)) return;
   }
   if (!g_ref.alg_idx.empty() && algebraicsForTransfer) fms_get_algebraics(g_ref.solver, algebraicsForTransfer);
+  FMS_API_END_VOID
 }

@@ -63,6 +63,7 @@ struct StepScheme {
 extern "C" int fms_schedule(double rc_end, double rc_width, double strang_end, double strang_width,
                             double heun_end, double heun_width, double ie_end, double ie_width, int max_calls,
                             double *out) {
+  FMS_API_BEGIN
   if (rc_width <= 0 || strang_width <= 0 || heun_width <= 0 || ie_width <= 0 || max_calls < 0)
     return fail(FMS_ERR_ARGUMENT, "time step widths must be positive");
   fms_host::StepScheme rc, st, he, ie;
@@ -93,4 +94,5 @@ extern "C" int fms_schedule(double rc_end, double rc_width, double strang_end, d
     t = rc.start + (double)step / rc.n * span;
   }
   return calls;
+  FMS_API_END
 }

@@ -34,6 +34,8 @@ struct ModelHH {
     const double v[NP] = {0, 120, 36, 0.3};
     return v[k];
   }
+  // every algebraic is computed (no parametersUsedAsAlgebraic)
+  static FMS_HD constexpr bool alg_is_param(int k) { return (void)k, false; }
 
   // rates r[0..NS) and algebraics a[0..NA) at state y; T is the arithmetic type
   template <class T, class E>

@@ -156,6 +156,14 @@ class FiberBatch:
         self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_ONLY_IF_STIMULATED, int(bool(on))),
                   "fms_set_tuning")
 
+    def set_point_buffer_order(self, width: int, nmj=None) -> None:
+        """onlyComputeIfHasBeenStimulated in the reference's sweep over buffers of
+        `width` nodes (compute.tpp:131-216); nmj: each fibre's stimulation
+        point (fibre-local); width 0 returns to the per-fibre rule"""
+        arr = np.ascontiguousarray(np.broadcast_to(nmj if nmj is not None else 0, (self.n_fibers,)), np.int32)
+        self._chk(self.lib.fms_set_point_buffer_order(self.handle, int(width), L.as_ptr(arr, L._ip)),
+                  "fms_set_point_buffer_order")
+
     def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None,
                    threads: int | None = None, graph_steps: int | None = None,
                    reaction_per_fibre: int | None = None, pair_halves: int | None = None,
@@ -192,8 +200,10 @@ class FiberBatch:
         return dict(captures=c.value, replays=r.value)
 
     def autotune(self, steps, n0D: int = 1, allow_split: bool = True) -> dict:
-        """Time every bit-identical schedule over the step records `steps` (n, 4)
-        on this batch's own states, restore the states, keep the fastest."""
+        """Time every candidate schedule over the step records `steps` (n, 4)
+        on this batch's own states, restore the states, keep the fastest of
+        those whose end state is bit-identical to the current schedule's
+        (`rejected` counts the others; their `candidates_ms` entry is -1)."""
         steps = np.ascontiguousarray(steps, dtype=np.float64).reshape(-1, 4)
         res = L.FmsTuneResult()
         ms = np.full(64, np.nan)
@@ -204,7 +214,8 @@ class FiberBatch:
         return dict(kernels=self.config["kernels"], steps_per_launch=res.steps_per_launch,
                     resident_gates=res.resident_gates, threads=res.threads, graph_steps=res.graph_steps,
                     reaction_per_fibre=res.reaction_per_fibre, reaction_threads=res.reaction_threads,
-                    ms_per_step=res.ms_per_step, candidates_ms=ms[:res.n_candidates].tolist())
+                    ms_per_step=res.ms_per_step, candidates_ms=ms[:res.n_candidates].tolist(),
+                    rejected=res.n_rejected)
 
     def record_event(self, slot: int) -> None:
         self._chk(self.lib.fms_record_event(self.handle, int(slot)), "fms_record_event")

@@ -187,3 +187,24 @@ def test_torch_autograd_on_device_equals_host_path():
     np.testing.assert_array_equal(tr.detach().cpu().numpy(), tr_host)
     expect = prob.backward(tr_host - target.cpu().numpy())
     np.testing.assert_allclose(th.grad.cpu().numpy(), expect, rtol=1e-12, atol=0)
+
+
+def test_two_outputs_in_one_loss():
+    """two MonodomainTraces outputs of one solver in one loss: each output's
+    gradient comes from its own forward (the solver holds the checkpoints of the
+    last forward only; the earlier output's backward recomputes its forward)"""
+    torch = pytest.importorskip("torch")
+    from opendihu_amd.adjoint import MonodomainTraces
+    prob, theta_true, theta0 = _problem(n_steps=300, interval=64)
+    dev = torch.device("cuda")
+    target = prob.forward_tensor(torch.tensor(theta_true, device=dev))
+    theta1 = theta0 * np.array([1.03, 0.98, 1.01, 0.97])
+    ta = torch.tensor(theta0, dtype=torch.float64, device=dev, requires_grad=True)
+    tb = torch.tensor(theta1, dtype=torch.float64, device=dev, requires_grad=True)
+    loss = 0.5 * ((MonodomainTraces.apply(ta, prob) - target) ** 2).sum() \
+        + 0.25 * ((MonodomainTraces.apply(tb, prob) - target) ** 2).sum()
+    loss.backward()
+    ga = prob.backward(prob.forward(theta0) - target.cpu().numpy())
+    gb = prob.backward(0.5 * (prob.forward(theta1) - target.cpu().numpy()))
+    np.testing.assert_allclose(ta.grad.cpu().numpy(), ga, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(tb.grad.cpu().numpy(), gb, rtol=1e-12, atol=0)

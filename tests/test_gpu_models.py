@@ -178,3 +178,44 @@ def test_device_rhs_and_heun_match_oracle(which, exp_mode):
         np.testing.assert_allclose(yh, yo, rtol=1e-10, atol=1e-12)
     # repeated device evaluation gives the same bits
     np.testing.assert_array_equal(L.eval_rhs(g, Y, Pm, 0.0, exp_mode), r)
+
+
+@pytest.mark.parametrize("model,kernels", [("hh", "split"), ("hh", "fused"), (SHORTEN, "split"), (SHORTEN, "fused"),
+                                           ("tk2014", "split")])
+def test_algebraics_for_transfer_match_oracle(model, kernels):
+    """algebraicsForTransfer: the algebraics of the last Heun step's second
+    evaluation (GEN/03_generator_vc.cpp:1131-1162), bit-identical to the
+    oracle's; an algebraic that is a parameter (parametersUsedAsAlgebraic, the
+    Shorten model's I_Stim = algebraic 32) is not stored (:1143-1146), so its
+    transfer values keep their initial zeros"""
+    from opendihu_amd.fibers import FiberBatch
+    if model == "hh":
+        mesh, stim = uneven(6, lo=300, hi=600, seed=12)
+        g = FiberBatch(**mesh, scheme="IE", thomas="serial", kernels=kernels)
+        o = OracleBatch(**mesh, scheme="IE")
+        idx = [0, 3, 4, 8]
+        for b in (g, o):
+            b.set_current_stimulation(**stim)
+        dt, steps, calls = 1e-3, 350, 2
+    else:
+        lib = _lib_or_skip(f"libfms_{model}.so")
+        mesh, stim = c2(n_fibers=4, n_nodes=301, length_cm=3.0, dt=2e-4, end_time=0.2, rate_hz=4000.0)
+        g = FiberBatch(**mesh, scheme="IE", thomas="serial", kernels=kernels, library=lib)
+        o = OracleBatch(**mesh, scheme="IE", model=model, threads=4)
+        idx = [0, 5, 32, 70] if model == SHORTEN else [0, 9, 44, 70]
+        for b in (g, o):
+            b.set_firing_table_stimulation(**stim)
+        dt, steps, calls = 2e-4, 250, 2
+    for b in (g, o):
+        b.set_algebraics_for_transfer(idx)
+    for c in range(calls):
+        for b in (g, o):
+            b.advance(c * steps * dt, dt, steps)
+        ga, oa = g.get_algebraics(), o.algebraics()
+        for q, k in enumerate(idx):
+            np.testing.assert_array_equal(ga[q], oa[q], err_msg=f"{model} algebraic {k} after call {c}")
+    if model == SHORTEN:
+        assert np.all(ga[2] == 0.0), "a parameter algebraic is not stored"
+    assert np.all(np.isfinite(ga)) and np.count_nonzero(ga) > 0
+    for k in range(g.n_states):
+        np.testing.assert_array_equal(g.get_state(k), o.get_state(k), err_msg=f"state {k}")

@@ -183,17 +183,22 @@ def test_schedule_driven_run_bit_exact():
 
 
 def test_reference_abi_matches_native():
-    """initializeArrays/computeMonodomain (the reference's dlopen boundary) against the oracle."""
+    """initializeArrays/computeMonodomain (the reference's dlopen boundary)
+    against the oracle: states and algebraics for transfer after every phase;
+    parameters and prefactor are taken from every call (gpu.tpp:481,490), a
+    call whose instances of one fibre carry different parameters is refused
+    and changes nothing"""
     import ctypes
     from opendihu_amd import _lib as L
     lib = L.load_library()
     F, n = 3, 301
-    d = L.FmsReferenceDims(n_fibers_to_compute=F, n_instances_per_fiber=n, n_parameters_per_instance=1,
+    N = F * n
+    d = L.FmsReferenceDims(n_fibers_to_compute=F, n_instances_per_fiber=n, n_parameters_per_instance=2,
                            firing_events_rows=2, firing_events_cols=100, frequency_jitter_cols=2,
-                           n_algebraics_for_transfer=1, n_states_for_transfer=2, scheme=0, precision=0, exp_mode=0)
+                           n_algebraics_for_transfer=2, n_states_for_transfer=2, scheme=0, precision=0, exp_mode=0)
     L.check(lib, lib.fms_reference_set_dims(ctypes.byref(d)), "set dims")
     y0 = np.array([-75, 0.05, 0.6, 0.325])
-    alg_idx = np.array([0], np.int32)
+    alg_idx = np.array([0, 6], np.int32)
     st_idx = np.array([0, 3], np.int32)
     fe = np.ones(200, np.uint8)
     jitter = np.array([0.1, -0.1] * F)
@@ -208,24 +213,53 @@ def test_reference_abi_matches_native():
                          fe.tobytes(), jitter.ctypes.data_as(dp), mu.ctypes.data_as(ip), nmj.ctypes.data_as(ip),
                          last.ctypes.data_as(dp), freq.ctypes.data_as(dp), rep.ctypes.data_as(dp),
                          beg.ctypes.data_as(dp))
-    params = np.zeros(F * n, np.float32)
+    # parameters[p*N + instance]: i_Stim = 0 and g_Na = 120 on every node
+    params = np.zeros(2 * N, np.float32)
+    params[N:] = 120.0
     lengths = np.full(F * (n - 1), 0.01, np.float32)
-    states_out = np.zeros(2 * F * n)
-    alg_out = np.zeros(F * n)
+    states_out = np.zeros(2 * N)
+    alg_out = np.zeros(2 * N)
     fp = ctypes.POINTER(ctypes.c_float)
-    for k in range(1000):
-        lib.computeMonodomain(params.ctypes.data_as(fp), alg_out.ctypes.data_as(dp), states_out.ctypes.data_as(dp),
-                              lengths.ctypes.data_as(fp), k * 2e-3, 2e-3, 1, 1e-3, 1, 2e-3, 1, 0.007656, 20.0)
-    assert L.last_error(lib) == ""
     o = OracleBatch(n_nodes=[n] * F, element_lengths=lengths.astype(np.float64), prefactor=0.007656, scheme="IE")
     o.set_firing_table_stimulation(nmj - 1, nmj + 2, fe.reshape(2, 100), mu, freq, rep, beg,
                                    jitter_lists=[[0.1, -0.1]] * F, last_check_time=last)
-    for k in range(1000):
-        o.advance(k * 2e-3, 2e-3, 1, 1e-3, 1, 2e-3, 1)
-    err = _rel_err(states_out[: F * n], o.vm())
-    assert err <= REL_TOL, err
-    assert _rel_err(states_out[F * n:], o.get_state(3)) <= REL_TOL
-    assert np.max(o.vm()) > 0.0 or np.max(states_out[: F * n]) > -80
+    o.set_algebraics_for_transfer(alg_idx)
+
+    def run(k0, k1, pf):
+        for k in range(k0, k1):
+            lib.computeMonodomain(params.ctypes.data_as(fp), alg_out.ctypes.data_as(dp),
+                                  states_out.ctypes.data_as(dp), lengths.ctypes.data_as(fp), k * 2e-3, 2e-3, 1, 1e-3,
+                                  1, 2e-3, 1, pf, 20.0)
+            assert L.last_error(lib) == "", L.last_error(lib)
+            o.advance(k * 2e-3, 2e-3, 1, 1e-3, 1, 2e-3, 1)
+
+    def check(phase):
+        assert _rel_err(states_out[:N], o.vm()) <= REL_TOL, phase
+        assert _rel_err(states_out[N:], o.get_state(3)) <= REL_TOL, phase
+        alg = o.algebraics()
+        for q in range(2):
+            assert _rel_err(alg_out[q * N:(q + 1) * N], alg[q]) <= 1e-9, (phase, q)
+
+    run(0, 600, 0.007656)
+    check("phase 1")
+    assert np.max(o.vm()) > 0.0, "the stimulated fibres fire"
+    # a coupled solver changes the prefactor and fibre 1's parameters
+    params[n:2 * n] = 2.5
+    params[N + n:N + 2 * n] = 100.0
+    o.pf[:] = 0.009
+    o.params[1, 0], o.params[1, 1] = 2.5, 100.0
+    run(600, 1000, 0.009)
+    check("phase 2")
+    # per-node parameters inside one fibre are refused; the states stay
+    before = states_out.copy()
+    params[N + 2 * n + 7] = 110.0
+    lib.computeMonodomain(params.ctypes.data_as(fp), alg_out.ctypes.data_as(dp), states_out.ctypes.data_as(dp),
+                          lengths.ctypes.data_as(fp), 1000 * 2e-3, 2e-3, 1, 1e-3, 1, 2e-3, 1, 0.009, 20.0)
+    assert "differs between nodes of fibre 2" in L.last_error(lib)
+    np.testing.assert_array_equal(states_out, before)
+    params[N + 2 * n + 7] = 120.0
+    run(1000, 1100, 0.009)
+    check("phase 3")
 
 
 @pytest.mark.parametrize("resident", [0, 1])
@@ -274,3 +308,37 @@ def test_only_compute_if_stimulated(kernels):
     off = np.concatenate([[0], np.cumsum(mesh["n_nodes"])])
     untouched = [f for f in range(12) if np.all(gates[off[f]:off[f + 1]] == 0.05)]
     assert 0 < len(untouched) < 12, "some fibres must still be unstimulated at t = 0.8 ms"
+
+
+@pytest.mark.parametrize("kernels,n0D", [("split", 1), ("split", 3), ("fused", 1)])
+def test_only_compute_if_stimulated_point_buffer_order(kernels, n0D):
+    """onlyComputeIfHasBeenStimulated in the reference's point-buffer order
+    (compute.tpp:131-216) against the oracle's sweep over buffers of four nodes:
+    equal fibres of 301 nodes (buffers straddle fibre ends, each taking its
+    first node's fibre), the stimulation checked in the centre buffer only, and
+    in the half step of a fibre's first stimulation the buffers before the
+    centre skip the half while those after it compute all of it. Bit-exact;
+    the per-fibre rule gives other states on the same inputs."""
+    F, n = 10, 301
+    mesh = dict(n_nodes=np.full(F, n, np.int32), element_lengths=np.full(F * (n - 1), 0.01),
+                prefactor=np.full(F, 0.007656))
+    nmj = np.full(F, n // 2, np.int32)
+    dt = 2e-3
+    g = _gpu_batch(mesh, scheme="IE", kernels=kernels, thomas="serial")
+    o = OracleBatch(**mesh, scheme="IE")
+    o_fibre = OracleBatch(**mesh, scheme="IE")
+    for b in (g, o, o_fibre):
+        _firing_setup(b, mesh, width=4)
+        b.set_only_compute_if_stimulated(True)
+        b.enable_spike_detection(0.0)
+    g.set_point_buffer_order(4, nmj)
+    o.enable_vc_sweep(nmj, width=4, equilibrium_skip=False)
+    for k in range(3):
+        for b in (g, o, o_fibre):
+            b.advance(k * 200 * dt, dt, 200, dt0D=dt / 2 / n0D, n0D=n0D)
+        np.testing.assert_array_equal(g.get_states(), o.get_states(), err_msg=f"call {k}")
+    np.testing.assert_array_equal(g.spikes()[0], o.spikes()[0])
+    assert (o.spikes()[1] > 0).sum() > 0, "fibres must fire"
+    assert not np.array_equal(o.get_states(), o_fibre.get_states()), "the sweep order must matter here"
+    cnt = o.vc_counters()
+    assert cnt["skipped_not_stimulated"] > 0 and cnt["computed"] > 0

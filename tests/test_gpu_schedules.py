@@ -162,6 +162,7 @@ def test_autotune_keeps_state_and_bits():
     for a, b in zip(before[2], after[2]):
         np.testing.assert_array_equal(a, b)
     assert len(choice["candidates_ms"]) >= 3 and all(np.isfinite(choice["candidates_ms"]))
+    assert choice["rejected"] == 0 and min(choice["candidates_ms"]) > 0, choice
     for b in (ref, tuned):
         b.advance(300 * dt, dt, 700)
     np.testing.assert_array_equal(ref.get_states(), tuned.get_states())
@@ -204,3 +205,22 @@ def test_paired_halves_equal_unpaired(thomas):
         ea = sorted(zip(ref[2][1].tolist(), ref[2][0].tolist()))
         ga = sorted(zip(run[2][1].tolist(), run[2][0].tolist()))
         assert ea == ga and len(ea) > 0, key
+
+
+def test_autotune_rejects_schedules_with_other_bits():
+    """the autotuner compares every candidate's end state with the solver's own
+    schedule's: a 128-thread fused schedule with the partitioned solve
+    partitions each fibre differently from every 256-thread candidate (a few ulp
+    apart), so all candidates are rejected and the solver keeps its schedule"""
+    mesh, stim = uneven(16, lo=300, hi=900, seed=5)
+    g = _batch(mesh, scheme="CN", kernels="fused", thomas="partitioned")
+    g.set_tuning(threads=128)
+    g.set_current_stimulation(**stim)
+    g.advance(0.0, 1e-3, 400)
+    dt = 1e-3
+    trial = np.stack([np.arange(40) * dt + 0.4, np.full(40, dt), np.full(40, dt / 2), np.full(40, dt)], 1)
+    choice = g.autotune(trial, allow_split=True)
+    print("autotune choice", choice)
+    assert choice["rejected"] == len(choice["candidates_ms"]) >= 3, choice
+    assert all(m == -1.0 for m in choice["candidates_ms"]) and choice["ms_per_step"] == -1.0
+    assert choice["kernels"] == "fused" and choice["threads"] == 128
