@@ -116,6 +116,19 @@ int fms_eval_rhs(const double *states, const double *params, double t, int n, in
    [n_states][n] in and out (host buffers). Synchronous. */
 int fms_eval_heun(double *states, const double *params, double t0, double dt, int n_steps, int n, int exp_mode);
 
+/* CellML.initializeStatesToEquilibrium (core/src/cellml/03_cellml_adapter.tpp:446-520):
+   classical RK4 of one cell (parameters [n_params], libm exp/log) from `states`
+   with step dt (initializeStatesToEquilibriumTimestepWidth, default 1e-4) until
+   the largest |dy/dt| of an RK4 step is below 1e-5; a step whose increment
+   exceeds 1e4 restarts from the initial values with half the step. states
+   [n_states] in and out (the last iterate if the search fails); iterations,
+   max_increment and dt_out (may be NULL) receive the iterations taken, the last
+   largest increment and the final step. Returns FMS_ERR_STATE if no equilibrium
+   was found within max_iterations (the reference uses 1e7). Runs on the calling
+   host thread: one serial recurrence, once, before the first step. */
+int fms_equilibrium_states(double *states, const double *params, double dt, long long max_iterations,
+                           long long *iterations, double *max_increment, double *dt_out);
+
 int fms_create(const FmsConfig *config, FmsSolver **solver);
 int fms_destroy(FmsSolver *solver);
 int fms_set_stimulation(FmsSolver *solver, const FmsStimulation *stim);

@@ -18,6 +18,7 @@ FMS_EXP_APPROXIMATE, FMS_EXP_LIBM = 0, 1
 FMS_THOMAS_SERIAL, FMS_THOMAS_PARTITIONED = 0, 1
 FMS_KERNELS_SPLIT, FMS_KERNELS_FUSED = 0, 1
 FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
+FMS_OK, FMS_ERR_ARGUMENT, FMS_ERR_HIP, FMS_ERR_STATE, FMS_ERR_UNSUPPORTED, FMS_ERR_INTERNAL = 0, -1, -2, -3, -4, -5
 FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES, FMS_TUNE_ONLY_IF_STIMULATED = 1, 2, 3
 FMS_TUNE_THREADS, FMS_TUNE_GRAPH_STEPS, FMS_TUNE_REACTION_PER_FIBRE, FMS_TUNE_PROFILE_KERNELS = 4, 5, 6, 7
 FMS_TUNE_PAIR_HALVES, FMS_TUNE_REACTION_THREADS = 8, 9
@@ -110,7 +111,7 @@ EXPORTED_SYMBOLS = [
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
     "fms_eval_rhs", "fms_eval_heun", "fms_adjoint_forward_device", "fms_adjoint_backward_device",
-    "fms_kernel_times", "fms_set_point_buffer_order",
+    "fms_kernel_times", "fms_set_point_buffer_order", "fms_equilibrium_states",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -164,6 +165,8 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_last_error.argtypes = [ctypes.c_char_p, i]
     lib.fms_set_tuning.argtypes = [v, i, i]
     lib.fms_set_point_buffer_order.argtypes = [v, i, _ip]
+    lib.fms_equilibrium_states.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_longlong,
+                                           ctypes.POINTER(ctypes.c_longlong), _dp, _dp]
     lib.fms_record_event.argtypes = [v, i]
     lib.fms_event_elapsed_ms.argtypes = [v, i, i, _dp]
     lib.fms_launch_counts.argtypes = [v, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
@@ -270,3 +273,16 @@ def schedule(lib: ctypes.CDLL, rc_end, rc_width, strang_end, strang_width, heun_
     lib.fms_schedule(rc_end, rc_width, strang_end, strang_width, heun_end, heun_width, ie_end, ie_width, n,
                      as_ptr(out, _dp))
     return out[:n]
+
+
+def equilibrium_states(lib: ctypes.CDLL, states, params, dt: float = 1e-4, max_iterations: int = 10_000_000) -> dict:
+    """initializeStatesToEquilibrium of one cell (core/src/cellml/03_cellml_adapter.tpp:446-520):
+    {states, iterations, max_increment, dt}; raises if no equilibrium was found"""
+    ns, _, nparams, _ = model_info(lib)
+    y = np.ascontiguousarray(states, np.float64).copy().reshape(ns)
+    p = np.ascontiguousarray(params if nparams > 0 else [0.0], np.float64)
+    it, inc, dt_out = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+    check(lib, lib.fms_equilibrium_states(as_ptr(y, _dp), as_ptr(p, _dp), float(dt), int(max_iterations),
+                                          ctypes.byref(it), ctypes.byref(inc), ctypes.byref(dt_out)),
+          "fms_equilibrium_states")
+    return dict(states=y, iterations=it.value, max_increment=inc.value, dt=dt_out.value)

@@ -768,6 +768,59 @@ int fms_eval_heun(double *states, const double *params, double t0, double dt, in
   FMS_API_END
 }
 
+// initializeStatesToEquilibrium of the CellML adapter
+// (core/src/cellml/03_cellml_adapter.tpp:446-520): classical RK4 on one cell
+// with the model's libm functions, the time left at 0 as there, until the
+// largest state increment per unit time is below 1e-5; an increment above 1e4
+// restarts from the initial values with half the step. The search is one
+// serial recurrence run once before the first step, so it runs on the host
+// core that calls it (the reference runs it on the host too); the model's
+// right-hand side is the same source the device kernels compile.
+int fms_equilibrium_states(double *states, const double *params, double dt, long long max_iterations,
+                           long long *iterations, double *max_increment, double *dt_out) {
+  FMS_API_BEGIN
+  if (!states || (Model::NP > 0 && !params) || !(dt > 0) || max_iterations < 0)
+    return fail(FMS_ERR_ARGUMENT, "fms_equilibrium_states: bad arguments");
+  constexpr int NS = Model::NS;
+  fms::FD u[NS], prev[NS], k1[NS], k2[NS], k3[NS], k4[NS], u2[NS], u3[NS], u4[NS], a[Model::NA];
+  fms::FD p[Model::NP > 0 ? Model::NP : 1];
+  for (int k = 0; k < Model::NP; k++) p[k] = fms::FD(params[k]);
+  for (int k = 0; k < NS; k++) u[k] = prev[k] = fms::FD(states[k]);
+  const double t = 0.0;
+  double incr = 0.0;
+  long long it = 0;
+  for (; it < max_iterations; it++) {
+    Model::template rhs<fms::FD, fms::ExpLibm>(u, p, t, k1, a);
+    for (int k = 0; k < NS; k++) u2[k] = fms::FD(u[k].v + dt / 2. * k1[k].v);
+    Model::template rhs<fms::FD, fms::ExpLibm>(u2, p, t + dt / 2., k2, a);
+    for (int k = 0; k < NS; k++) u3[k] = fms::FD(u[k].v + dt / 2. * k2[k].v);
+    Model::template rhs<fms::FD, fms::ExpLibm>(u3, p, t + dt / 2., k3, a);
+    for (int k = 0; k < NS; k++) u4[k] = fms::FD(u[k].v + dt * k3[k].v);
+    Model::template rhs<fms::FD, fms::ExpLibm>(u4, p, t + dt, k4, a);
+    incr = 0.0;
+    for (int k = 0; k < NS; k++) {
+      const double inc = (k1[k].v + 2 * k2[k].v + 2 * k3[k].v + k4[k].v) / 6.;
+      if (std::fabs(inc) > incr) incr = std::fabs(inc);
+      u[k] = fms::FD(u[k].v + dt * inc);
+    }
+    if (incr > 1e4) {
+      dt /= 2.0;
+      for (int k = 0; k < NS; k++) u[k] = prev[k];
+    }
+    if (incr < 1e-5) break;
+  }
+  for (int k = 0; k < NS; k++) states[k] = u[k].v;
+  if (iterations) *iterations = it;
+  if (max_increment) *max_increment = incr;
+  if (dt_out) *dt_out = dt;
+  if (!(incr < 1e-5))
+    return fail(FMS_ERR_STATE, "equilibrium values of the states were not found within " +
+                                   std::to_string(max_iterations) + " RK4 iterations (last increment " +
+                                   std::to_string(incr) + ", step " + std::to_string(dt) + ")");
+  return 0;
+  FMS_API_END
+}
+
 int fms_create(const FmsConfig *c, FmsSolver **out) {
   FMS_API_BEGIN
   if (!c || !out) return fail(FMS_ERR_ARGUMENT, "null argument");

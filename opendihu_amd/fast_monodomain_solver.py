@@ -23,10 +23,19 @@ core/src/specialized_solver/fast_monodomain_solver/fast_monodomain_solver_initia
     algebraicsForTransfer, stimulationLogFilename, meshName, libraryFilename
   ImplicitEuler | CrankNicolson: timeStepWidth; FiniteElementMethod: meshName, prefactor
   Meshes.<name>: nElements, physicalExtent, physicalOffset | nodePositions
+  CellML: initializeStatesToEquilibrium (False), initializeStatesToEquilibriumTimestepWidth (1e-4),
+    statesForTransfer ([0]), useSinglePrecision at RepeatedCall level (gpu back end: runs the
+    mixed form, FP32 gates with FP64 Vm and diffusion; a warning says so)
+  StrangSplitting.OutputWriter: {"format": "PythonFile", "outputInterval", "filename", "binary",
+    "fileNumbering"} (opendihu_amd.output)
+Keys that change nothing here (log keys, callbacks the fast solver does not call,
+boundary conditions of the nested solvers) are accepted silently; every other key
+this front end does not apply raises a warning that names it.
 Extension keys (ours): "vcVectorWidth" (stimulated lanes of the vc path, default 4),
 "neuromuscularJunctionSeed" (the reference draws NMJ positions unseeded),
 "kernels" ("fused" | "split"), "thomas" ("partitioned" | "serial"), "precision"
-("fp64" | "mixed").
+("fp64" | "mixed"), "pointBufferOrder" (True: onlyComputeIfHasBeenStimulated in the
+reference's sweep over buffers of vcVectorWidth nodes, compute.tpp:131-216).
 """
 from __future__ import annotations
 
@@ -41,6 +50,37 @@ from .settings import (element_lengths, gpu_stimulation_range, node_positions, p
                        parse_firing_times, vc_stimulation_range, write_stimulation_log)
 
 HH_CONSTANT_PARAMETERS = [2, 3, 4, 5]   # i_Stim, g_Na, g_K, g_L of hodgkin_huxley_1952
+
+# keys per settings block: applied here, or without effect on this path (logging,
+# callbacks and nested-solver options the fast solver does not use)
+_APPLIED = {
+    "RepeatedCall": {"timeStepWidth", "endTime", "MultipleInstances", "fiberDistributionFile", "firingTimesFile",
+                     "onlyComputeIfHasBeenStimulated", "disableComputationWhenStatesAreCloseToEquilibrium",
+                     "valueForStimulatedPoint", "neuromuscularJunctionRelativeSize", "useSinglePrecision",
+                     "vcVectorWidth", "neuromuscularJunctionSeed", "kernels", "thomas", "precision",
+                     "pointBufferOrder"},
+    "StrangSplitting": {"timeStepWidth", "endTime", "Term1", "Term2", "OutputWriter"},
+    "Heun": {"timeStepWidth", "CellML"},
+    "CellML": {"modelFilename", "optimizationType", "approximateExponentialFunction", "statesInitialValues",
+               "initializeStatesToEquilibrium", "initializeStatesToEquilibriumTimestepWidth",
+               "setSpecificStatesCallFrequency", "setSpecificStatesFrequencyJitter",
+               "setSpecificStatesRepeatAfterFirstCall", "setSpecificStatesCallEnableBegin", "additionalArgument",
+               "parametersUsedAsAlgebraic", "parametersUsedAsConstant", "parametersInitialValues",
+               "algebraicsForTransfer", "statesForTransfer", "stimulationLogFilename", "meshName", "libraryFilename"},
+    "Diffusion": {"timeStepWidth", "FiniteElementMethod"},
+    "FiniteElementMethod": {"meshName", "prefactor"},
+}
+_NO_EFFECT = {"logKey", "ranksAllComputedInstances", "nInstances", "ranks", "numberTimeSteps",
+              "logTimeStepWidthAsKey", "durationLogKey", "timeStepOutputInterval", "initialValues",
+              "dirichletBoundaryConditions", "dirichletOutputFilename", "inputMeshIsGlobal", "checkForNanInf",
+              "nAdditionalFieldVariables", "additionalSlotNames", "compilerFlags", "maximumNumberOfThreads",
+              "setSpecificStatesFunction", "setSpecificStatesCallInterval", "setSpecificParametersFunction",
+              "setSpecificParametersCallInterval", "handleResultFunction", "handleResultCallInterval",
+              "connectedSlotsTerm1To2", "connectedSlotsTerm2To1", "solverName", "slotName", "slotNames",
+              "connectorSlotsTerm1To2", "connectorSlotsTerm2To1", "OutputWriter", "solverType", "preconditionerType",
+              "relativeTolerance", "absoluteTolerance", "maxIterations", "dumpFilename", "dumpFormat",
+              "timeStepWidthRelativeTolerance", "timeStepWidthRelativeToleranceAsKey", "description",
+              "reuseResultsFromPreviousRun", "nElements", "inputMeshIsGlobal", "enableFiberDistribution"}
 
 
 def _list(v):
@@ -86,11 +126,28 @@ class FastMonodomainSolver:
                 fibres.append((strang, heun, key, b[key]))
         return fibres
 
+    def _check_keys(self, fibres):
+        """warn about every settings key that this front end neither applies nor
+        knows to be without effect on the fibre path"""
+        blocks = [("RepeatedCall", self.rc)]
+        for strang, heun, key, diff in fibres[:1]:
+            blocks += [("StrangSplitting", strang), ("Heun", heun), ("CellML", heun.get("CellML", {})),
+                       ("Diffusion", diff), ("FiniteElementMethod", diff.get("FiniteElementMethod", {}))]
+        for name, block in blocks:
+            unknown = sorted(k for k in block if k not in _APPLIED[name] and k not in _NO_EFFECT)
+            for k in unknown:
+                msg = f"{name}.{k} is not applied by the MI355X fibre solver"
+                if k == "mappings":
+                    msg += (" (give parametersUsedAsAlgebraic / parametersUsedAsConstant / statesForTransfer / "
+                            "algebraicsForTransfer as index lists instead)")
+                warnings.warn(msg, stacklevel=3)
+
     def initialize(self):
         if self.initialized:
             return
         cfg, rc = self.config, self.rc
         fibres = self._fibers()
+        self._check_keys(fibres)
         if not fibres:
             raise ValueError("no fibres in the settings")
         strang0, heun0, scheme_key, diff0 = fibres[0]
@@ -113,23 +170,33 @@ class FastMonodomainSolver:
 
         # fibres: meshes, element lengths, prefactors
         meshes = cfg.get("Meshes", {})
-        n_nodes, lengths, prefactors, fiber_no = [], [], [], []
+        n_nodes, lengths, prefactors, fiber_no, positions = [], [], [], [], []
         for k, (_, heun, _, diff) in enumerate(fibres):
             fem = diff["FiniteElementMethod"]
             mesh_name = fem.get("meshName", heun["CellML"].get("meshName"))
             pos = node_positions(meshes[mesh_name])
             n_nodes.append(len(pos))
+            positions.append(np.asarray(pos, np.float64).reshape(len(pos), -1))
             lengths.append(element_lengths(pos))
             prefactors.append(float(fem.get("prefactor", 1.0)))
             fiber_no.append(int(heun["CellML"].get("additionalArgument", k) or 0))
         self.n_nodes = np.array(n_nodes, np.int32)
+        self.node_positions = [np.pad(p, ((0, 0), (0, 3 - p.shape[1]))) if p.shape[1] < 3 else p[:, :3]
+                               for p in positions]
         self.fiber_no = np.array(fiber_no, np.int32)
         F = len(fibres)
 
         # parameters per fibre
+        precision = rc.get("precision", "fp64")
+        if rc.get("useSinglePrecision", False):
+            # gpu.tpp:264-272 makes the whole gpu back end single precision; here
+            # the states are FP32 while Vm and the diffusion solve stay FP64
+            warnings.warn("useSinglePrecision: running the mixed form (FP32 gates, FP64 Vm and diffusion)",
+                          stacklevel=2)
+            precision = "mixed"
         self._batch_kw = dict(
             scheme=self.scheme,
-            precision=rc.get("precision", "fp64"),
+            precision=precision,
             exp_mode="approx" if cellml0.get("approximateExponentialFunction", True) else "libm",
             thomas=rc.get("thomas", "partitioned"),
             kernels=rc.get("kernels", "fused"),
@@ -147,6 +214,17 @@ class FastMonodomainSolver:
         for j, v in enumerate(p_init):
             if j < len(slots) and slots[j] is not None:
                 params[:, slots[j]] = v
+        if cellml0.get("initializeStatesToEquilibrium", False):
+            # 00_cellml_adapter_base.tpp:148-151, 03_cellml_adapter.tpp:446-520
+            start = np.zeros(n_states)
+            L.check(lib, lib.fms_model_defaults(L.as_ptr(start, L._dp), None), "fms_model_defaults")
+            if y0:
+                start = np.asarray(y0, float)
+            eq = L.equilibrium_states(lib, start, params[0],
+                                      float(cellml0.get("initializeStatesToEquilibriumTimestepWidth", 1e-4)))
+            self.equilibrium = eq
+            y0 = list(eq["states"])
+            self._write_equilibrium_file(os.path.splitext(model_file)[0] if model_file else base, eq)
         self.batch = FiberBatch(self.n_nodes, np.concatenate(lengths), np.array(prefactors), parameters=params,
                                 initial_states=(None if not y0 else np.asarray(y0, float)), **self._batch_kw)
         self.parameters = params
@@ -183,6 +261,7 @@ class FastMonodomainSolver:
             self.batch.set_firing_table_stimulation(sb, se, table, mus, np.array(freq), np.array(rep),
                                                     np.array(beg), jitter_lists=jit, log_capacity=1 << 16)
             self.stim = dict(stim_begin=sb, stim_end=se, firing_events=table, motor_unit=mus)
+            self.stim_nmj = nmj
         if rc.get("onlyComputeIfHasBeenStimulated", True):
             self.batch.set_only_compute_if_stimulated(True)
         if rc.get("disableComputationWhenStatesAreCloseToEquilibrium", True):
@@ -191,6 +270,19 @@ class FastMonodomainSolver:
         alg = _list(cellml0.get("algebraicsForTransfer", []))
         if alg:
             self.batch.set_algebraics_for_transfer(alg)
+        self.algebraics_for_transfer = alg
+        self.states_for_transfer = _list(cellml0.get("statesForTransfer", [0])) or [0]
+        if any(not 0 <= k < n_states for k in self.states_for_transfer):
+            raise ValueError(f"statesForTransfer {self.states_for_transfer}: state numbers are 0..{n_states - 1}")
+        if rc.get("pointBufferOrder", False):
+            nmj_ = self.stim_nmj if self.stim is not None else self.n_nodes // 2
+            self.batch.set_point_buffer_order(int(rc.get("vcVectorWidth", 4)), nmj_)
+        self.output_writers = [w for w in _list(strang0.get("OutputWriter", [])) if isinstance(w, dict)]
+        for w in self.output_writers:
+            if w.get("format") != "PythonFile":
+                warnings.warn(f"OutputWriter format {w.get('format')!r} is not written (PythonFile only)",
+                              stacklevel=2)
+        self.output_writers = [w for w in self.output_writers if w.get("format") == "PythonFile"]
         self.stimulation_log_file = cellml0.get("stimulationLogFilename", "out/stimulation.log")
 
         # time stepping
@@ -201,6 +293,20 @@ class FastMonodomainSolver:
                            float(strang0["timeStepWidth"]), float(diff0["timeStepWidth"]))
         self.schedule = sched
         self.initialized = True
+
+    def _write_equilibrium_file(self, stem, eq):
+        """<model>_equilibrium_values.txt as the reference writes it (03_cellml_adapter.tpp:527-560)"""
+        path = self._path(f"{stem}_equilibrium_values.txt")
+        try:
+            with open(path, "w") as f:
+                f.write(f"// equilibrium values of the states, {eq['iterations']} RK-4 iterations, dt: {eq['dt']}\n"
+                        f"// Maximum du/dt = {eq['max_increment']}\n\n")
+                for k, v in enumerate(eq["states"]):
+                    f.write(f"state[{k}] = {v!r};\n")
+                f.write("\n  Line to copy for settings:\n  \"statesInitialValues\": ["
+                        + ", ".join(repr(float(v)) for v in eq["states"]) + "],\n")
+        except OSError as e:
+            warnings.warn(f"could not write {path}: {e}", stacklevel=3)
 
     def _model_library(self, model_file, base, p_alg, p_const):
         if base == "hodgkin_huxley_1952":
@@ -234,9 +340,30 @@ class FastMonodomainSolver:
         for t0, dts, ns, dt0, _, dt1, _ in s:
             for k in range(int(ns)):
                 rows.append((t0 + k * dts, dts, dt0, dt1))
-        self.batch.advance_schedule(np.array(rows), int(n0[0]))
+        rows = np.array(rows)
+        writers = self._writers()
+        if not writers:
+            self.batch.advance_schedule(rows, int(n0[0]))
+        else:
+            # output after every outputInterval split steps, and the initial values
+            interval = min(w.interval for w in writers)
+            for w in writers:
+                w.write(self, step=0, time=float(rows[0, 0]))
+            for k0 in range(0, len(rows), interval):
+                chunk = rows[k0:k0 + interval]
+                self.batch.advance_schedule(chunk, int(n0[0]))
+                step = k0 + len(chunk)
+                for w in writers:
+                    if step % w.interval == 0 or step == len(rows):
+                        w.write(self, step=step, time=float(chunk[-1, 0] + chunk[-1, 1]))
         self.batch.synchronize()
         self.current_time = float(s[-1, 0] + s[-1, 1] * s[-1, 2])
+
+    def _writers(self):
+        from .output import PythonFileWriter
+        if not hasattr(self, "_writer_objs"):
+            self._writer_objs = [PythonFileWriter.from_settings(w, self._path) for w in self.output_writers]
+        return self._writer_objs
 
     def run(self):
         self.initialize()
@@ -249,6 +376,19 @@ class FastMonodomainSolver:
         v = self.batch.vm()
         off = np.concatenate([[0], np.cumsum(self.n_nodes)])
         return [v[off[f]:off[f + 1]] for f in range(len(self.n_nodes))]
+
+    def transfer(self) -> dict:
+        """statesForTransfer and algebraicsForTransfer after the last call, per
+        fibre (the vc back end's updateFiberData, communication.tpp:237-296)"""
+        off = np.concatenate([[0], np.cumsum(self.n_nodes)])
+        states = [self.batch.get_state(k) for k in self.states_for_transfer]
+        out = dict(states={k: [v[off[f]:off[f + 1]] for f in range(len(self.n_nodes))]
+                           for k, v in zip(self.states_for_transfer, states)})
+        if self.algebraics_for_transfer:
+            a = self.batch.get_algebraics()
+            out["algebraics"] = {k: [a[q, off[f]:off[f + 1]] for f in range(len(self.n_nodes))]
+                                 for q, k in enumerate(self.algebraics_for_transfer)}
+        return out
 
     def write_stimulation_log(self, path):
         times, fibers = self.batch.stimulation_log()
