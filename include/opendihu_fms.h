@@ -143,6 +143,20 @@ int fms_advance(FmsSolver *solver, double start_time, double dt_split, int n_spl
 int fms_advance_schedule(FmsSolver *solver, const double *steps, int n_steps, int n0D);
 int fms_synchronize(FmsSolver *solver);
 
+/* Allocation-free variant of fms_advance_schedule (no counterpart in the
+   reference; for callers that cannot allocate device memory: serving loops,
+   harnesses that forbid hipMalloc). Calls no hipMalloc, hipFree or other
+   allocating routine: the step table and the per-fibre stimulation flags of the
+   call live in a fixed 4 MiB device scratch array that each GPU holds until the
+   process ends (32 bytes per split step plus 2 bytes per fibre and Heun step;
+   larger calls fail with FMS_ERR_UNSUPPORTED). Graph replay is off for the call
+   (capturing would allocate); the serial solve with the split kernels needs the
+   scratch fms_create or an earlier fms_advance made. Returns when the call's
+   kernels are done, so calls made one after another on a GPU never overlap;
+   calls from several host threads at once on one GPU share its scratch and
+   race. Same results as fms_advance_schedule. */
+int fms_advance_schedule_noalloc(FmsSolver *solver, const double *steps, int n_steps, int n0D);
+
 int fms_get_states(FmsSolver *solver, int state_no, double *out /* [n_total] */);
 int fms_set_states(FmsSolver *solver, int state_no, const double *in /* [n_total] */);
 int fms_get_algebraics(FmsSolver *solver, double *out /* [n_alg_transfer*n_total] */);

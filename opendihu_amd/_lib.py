@@ -111,7 +111,7 @@ EXPORTED_SYMBOLS = [
     "fms_set_parameters", "fms_set_sensors", "fms_adjoint_forward", "fms_adjoint_backward",
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
     "fms_eval_rhs", "fms_eval_heun", "fms_adjoint_forward_device", "fms_adjoint_backward_device",
-    "fms_kernel_times", "fms_set_point_buffer_order", "fms_equilibrium_states",
+    "fms_kernel_times", "fms_set_point_buffer_order", "fms_equilibrium_states", "fms_advance_schedule_noalloc",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -165,6 +165,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_last_error.argtypes = [ctypes.c_char_p, i]
     lib.fms_set_tuning.argtypes = [v, i, i]
     lib.fms_set_point_buffer_order.argtypes = [v, i, _ip]
+    lib.fms_advance_schedule_noalloc.argtypes = [v, _dp, i, i]
     lib.fms_equilibrium_states.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_longlong,
                                            ctypes.POINTER(ctypes.c_longlong), _dp, _dp]
     lib.fms_record_event.argtypes = [v, i]

@@ -105,6 +105,7 @@ struct FmsSolver {
   int nmax = 0;
   std::vector<int> h_off;     // [F+1]
   std::vector<int> h_order;   // fibres longest first
+  int device = 0;             // the device current at fms_create
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // adjoint: recompute of the next segment beside the reverse of this one
   hipEvent_t adj_ev[4] = {};
@@ -320,10 +321,9 @@ int upload_steps(FmsSolver *s, const double *steps, int K) {
     s->steps_cap = 4 * (K + 1);
     if (dalloc(&s->d_steps, (size_t)s->steps_cap)) return FMS_ERR_HIP;
   }
-  std::vector<double> tab(4 * (size_t)(K + 1));
-  memcpy(tab.data(), steps, sizeof(double) * 4 * K);
-  for (int q = 0; q < 4; q++) tab[4 * (size_t)K + q] = steps[4 * (size_t)(K - 1) + q];
-  HIP_TRY(hipMemcpyAsync(s->d_steps, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_steps, steps, 4 * (size_t)K * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_steps + 4 * (size_t)K, steps + 4 * (size_t)(K - 1), 4 * sizeof(double),
+                         hipMemcpyHostToDevice, s->stream));
   hipLaunchKernelGGL(fms::k_cursor_set, dim3(1), dim3(64), 0, s->stream, s->d_cursor, 0LL, s->step_counter,
                      (const double *)s->d_steps);
   HIP_TRY(hipGetLastError());
@@ -663,6 +663,28 @@ int dispatch(FmsSolver *s, const double *h_steps, int K, int n0D) {
 
 size_t gate_bytes(const FmsSolver *s) { return s->precision == FMS_FP64 ? sizeof(double) : sizeof(float); }
 
+// Device scratch of the allocation-free advance (fms_advance_schedule_noalloc):
+// the split-step table and the per-fibre flags of one call. Each GPU the
+// library runs on holds its own copy until the process ends; host code uses the
+// address hipGetSymbolAddress returns for the current device (noalloc_ptr), never
+// the array's name, which in host code is only a stand-in.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)4 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+unsigned char *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// set-up, with the device current: fetch this device's scratch address once
+int noalloc_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(FMS_ERR_UNSUPPORTED, "device number out of range");
+  if (g_noalloc_ptr[device]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(FMS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_noalloc_ptr[device] = static_cast<unsigned char *>(ptr);
+  return 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -878,6 +900,14 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
   auto chk = [&](int r) {
     if (r && !rc) rc = r;
   };
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return fail(FMS_ERR_HIP, "hipGetDevice failed");
+  }
+  if (int r = noalloc_init(s->device)) {
+    delete s;
+    return r;
+  }
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return fail(FMS_ERR_HIP, "hipStreamCreate failed");
@@ -1066,6 +1096,53 @@ int fms_advance_schedule(FmsSolver *s, const double *steps, int n_steps, int n0D
   FMS_API_BEGIN
   if (!s || (!steps && n_steps > 0)) return fail(FMS_ERR_ARGUMENT, "null argument");
   return dispatch(s, steps, n_steps, n0D);
+  FMS_API_END
+}
+
+int fms_advance_schedule_noalloc(FmsSolver *s, const double *steps, int n_steps, int n0D) {
+  FMS_API_BEGIN
+  if (!s || (!steps && n_steps > 0)) return fail(FMS_ERR_ARGUMENT, "null argument");
+  if (n_steps <= 0) return 0;
+  if (n0D < 1) return fail(FMS_ERR_ARGUMENT, "n0D must be >= 1");
+  unsigned char *base = s->device >= 0 && s->device < NOALLOC_MAX_DEVICES ? g_noalloc_ptr[s->device] : nullptr;
+  if (!base) return fail(FMS_ERR_STATE, "no scratch address for the solver's device");
+  // the table (n_steps + 1 records of 4 doubles) and the flags (2 F n0D bytes)
+  const size_t table_bytes = ((size_t)(n_steps + 1) * 4 * sizeof(double) + 255) / 256 * 256;
+  const size_t flag_bytes = (size_t)2 * s->F * n0D;
+  if (table_bytes + flag_bytes > NOALLOC_SCRATCH_BYTES)
+    return fail(FMS_ERR_UNSUPPORTED,
+                "fms_advance_schedule_noalloc: " + std::to_string(n_steps) + " steps of " + std::to_string(s->F) +
+                    " fibres need " + std::to_string(table_bytes + flag_bytes) + " bytes of scratch, more than the " +
+                    std::to_string(NOALLOC_SCRATCH_BYTES) + " bytes it has (32 bytes per step plus 2 bytes per "
+                    "fibre and Heun step); split the call");
+  // schedules that would allocate are refused: the serial solve's scratch is
+  // made at fms_create only for the split kernels
+  const bool split_path = s->kernels == FMS_KERNELS_SPLIT || s->vc_width || s->d_alg;
+  if (s->thomas == FMS_THOMAS_SERIAL && split_path && !s->d_cA)
+    return fail(FMS_ERR_UNSUPPORTED, "fms_advance_schedule_noalloc: this schedule needs the serial solve's scratch; "
+                                     "call fms_advance_schedule once first");
+  double *keep_steps = s->d_steps;
+  unsigned char *keep_flags = s->d_flags;
+  const int keep_steps_cap = s->steps_cap, keep_flags_cap = s->flags_cap, keep_graph = s->tune_graph;
+  s->d_steps = reinterpret_cast<double *>(base);
+  s->steps_cap = (int)(table_bytes / sizeof(double));
+  s->d_flags = base + table_bytes;
+  s->flags_cap = (int)flag_bytes;
+  s->tune_graph = 0;   // a graph capture would allocate: eager launches
+  const int rc = dispatch(s, steps, n_steps, n0D);
+  s->d_steps = keep_steps;
+  s->d_flags = keep_flags;
+  s->steps_cap = keep_steps_cap;
+  s->flags_cap = keep_flags_cap;
+  s->tune_graph = keep_graph;
+  if (rc) {
+    (void)hipStreamSynchronize(s->stream);
+    return rc;
+  }
+  // return once the kernels are done with the scratch, so that calls made one
+  // after another on this device never overlap
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return 0;
   FMS_API_END
 }
 
