@@ -258,7 +258,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   const int n = fb.fiber_off[f + 1] - off;
   const double *h = fb.elem_len + off - f;
   const double pf = fb.prefactor[f];
-  double *red = sh;                          // partitioned solve (pcr_lds_doubles)
+  double *red = sh;                          // partitioned solve's block interfaces (pcr_lds_doubles)
   double *lw = sh + pcr_lds_doubles<T>();    // n: lambda_V, then w
   double *lv = lw + n;       // n: the diffusion's V (IE: V^b, CN: V^a + V^b), for the neighbours
   double p[4];

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   const int n = fb.fiber_off[f + 1] - off;
   const double *h = fb.elem_len + off - f;
   const double pf = fb.prefactor[f];
-  double *red = sh;   // PCR system (pcr_lds_doubles), or the serial solve's cA/dA (16*T doubles)
+  double *red = sh;   // block interfaces of the partitioned solve (pcr_lds_doubles), or the serial solve's cA/dA (16*T doubles)
   double *lv = sh + (serial ? 16 * T : pcr_lds_doubles<T>());   // n doubles: Vm
   unsigned char *sflags = (unsigned char *)(lv + n);   // 2*n0D flags
 

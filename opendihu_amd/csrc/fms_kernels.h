@@ -502,27 +502,37 @@ __global__ __launch_bounds__(64) void k_thomas_serial(DevFibers fb, double *vm, 
 }
 
 // ----------------------------------------------------------------------------
-// Partitioned tridiagonal solve of one fibre by a group of T threads, Vm in LDS.
-//
-// Each of P <= T threads owns a contiguous chunk of >= 2 rows. Within a chunk a
-// forward and a backward elimination express every row through the chunk's
-// first and last unknowns; the first/last rows of all chunks form a tridiagonal
-// system of 2T rows (identity rows for idle threads), solved by cyclic
-// reduction (PCR) in LDS; then every thread recovers its interior unknowns.
+// Partitioned tridiagonal solve of one fibre by a group of T threads (whole
+// waves), Vm in LDS. Three levels:
+//  1. Each of P <= T threads owns a contiguous chunk of >= 2 rows. A forward and
+//     a backward elimination within the chunk (rows scaled to a unit diagonal)
+//     express every row through the chunk's first and last unknowns; the
+//     chunks' first and last rows form a tridiagonal system of 2T rows
+//     (identity rows for idle threads), rows 2t and 2t+1 held by thread t.
+//  2. Each wave's block of 128 rows is solved in registers for its right-hand
+//     side and for its two couplings to the neighbouring blocks' unknowns xL
+//     (the previous block's last) and xR (the next block's first): one odd-even
+//     elimination step, then parallel cyclic reduction over the 64 even rows
+//     with cross-lane shuffles (no LDS, no barrier). Every unknown of the block
+//     becomes d - v xL - u xR.
+//  3. The blocks' first and last unknowns form a chain of 2 W unknowns (W waves)
+//     that every thread solves itself after one barrier; each thread then
+//     recovers its two rows and its chunk's interior.
 // Rows are assembled exactly as the reference does; only the elimination order
 // differs from its serial recurrence. The matrix is strictly diagonally
 // dominant for every dt (mass term), so no pivoting is needed.
 //
 // lv: n doubles of Vm in LDS (overwritten with the solution)
-// red: 8*2T doubles of LDS scratch
+// red: pcr_lds_doubles<T>() doubles of LDS scratch (6 per wave)
 // Requires n >= 2 and ceil(n / min(T, n/2)) <= MMAX. Ends with a barrier.
-// ----------------------------------------------------------------------------
 // GIVEN_RHS: lv holds the right-hand side itself (adjoint solves) instead of the
 // Vm from which the right-hand side B*Vm is assembled.
+// ----------------------------------------------------------------------------
 template <int T, int MMAX, bool GIVEN_RHS = false>
 __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double *h, double prefactor, double dt,
                                                 bool ie, double *red, int tid) {
-  constexpr int R = 2 * T;
+  static_assert(T % 64 == 0 && T >= 64 && T <= 1024, "the solve works on whole waves");
+  constexpr int W = T / 64;
   const int P = n / 2 < T ? n / 2 : T;
   double aa[MMAX], cc[MMAX], dd[MMAX];
   int lo = 0, cnt = 0;
@@ -569,55 +579,101 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     for (int k = 0; k < MMAX; k++)
       if (k == cnt - 1) { la = aa[k]; lc = cc[k]; ld = dd[k]; }
   }
-  // reduced system of the 2P chunk end rows, each row scaled to a unit
-  // diagonal: a x_{i-1} + x_i + c x_{i+1} = d. PCR keeps the rows normalised
-  // (one reciprocal per row and level), so 6 arrays of R doubles suffice
-  double *A0 = red, *C0 = red + R, *D0 = red + 2 * R;
-  double *A1 = red + 3 * R, *C1 = red + 4 * R, *D1 = red + 5 * R;
-  A0[2 * tid] = fa; C0[2 * tid] = fc; D0[2 * tid] = fd;
-  A0[2 * tid + 1] = la; C0[2 * tid + 1] = lc; D0[2 * tid + 1] = ld;
-  __syncthreads();
-  for (int s = 1; s < R; s <<= 1) {
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int i = tid + q * T;
-      const int im = i - s, ip = i + s;
-      double na = 0, nc = 0, nb = 1.0, nd = D0[i];
-      if (im >= 0) {
-        const double al = -A0[i];
-        na = al * A0[im];
-        nb += al * C0[im];
-        nd += al * D0[im];
-      }
-      if (ip < R) {
-        const double ga = -C0[i];
-        nc = ga * C0[ip];
-        nb += ga * A0[ip];
-        nd += ga * D0[ip];
-      }
-      const double r = 1.0 / nb;
-      A1[i] = na * r; C1[i] = nc * r; D1[i] = nd * r;
-    }
-    __syncthreads();
-    double *t;
-    t = A0; A0 = A1; A1 = t;
-    t = C0; C0 = C1; C1 = t;
-    t = D0; D0 = D1; D1 = t;
+
+  // level 2. Row convention: a x_prev + x + c x_next + v xL + u xR = d.
+  // Even row (chunk first) E, odd row (chunk last) O of this lane; the block's
+  // couplings to the outside move to the v and u columns.
+  const int lane = tid & 63, wave = tid >> 6;
+  double aE = fa, cE = fc, dE = fd, vE = 0, uE = 0;
+  double aO = la, cO = lc, dO = ld, vO = 0, uO = 0;
+  if (lane == 0) { vE = aE; aE = 0; }
+  if (lane == 63) { uO = cO; cO = 0; }
+  // odd-even step: the even row eliminates x_{2l-1} (odd row of lane l-1) and
+  // x_{2l+1} (its own odd row); the even rows then couple even unknowns only
+  double A, C, D, V, U;
+  {
+    const int src = (lane + 63) & 63;
+    double pa = __shfl(aO, src), pc = __shfl(cO, src), pd = __shfl(dO, src), pv = __shfl(vO, src),
+           pu = __shfl(uO, src);
+    if (lane == 0) pa = pc = pd = pv = pu = 0;
+    const double al = -aE, ga = -cE;
+    A = al * pa;
+    C = ga * cO;
+    const double r = 1.0 / (1.0 + al * pc + ga * aO);
+    D = r * (dE + al * pd + ga * dO);
+    V = r * (vE + al * pv + ga * vO);
+    U = r * (uE + al * pu + ga * uO);
+    A *= r;
+    C *= r;
   }
+  // parallel cyclic reduction over the 64 even rows of the wave
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const bool hm = lane >= s, hp = lane + s < 64;
+    const int im = (lane - s) & 63, ip = (lane + s) & 63;
+    const double Am = __shfl(A, im), Cm = __shfl(C, im), Dm = __shfl(D, im), Vm = __shfl(V, im), Um = __shfl(U, im);
+    const double Ap = __shfl(A, ip), Cp = __shfl(C, ip), Dp = __shfl(D, ip), Vp = __shfl(V, ip), Up = __shfl(U, ip);
+    const double al = hm ? -A : 0.0, ga = hp ? -C : 0.0;
+    const double r = 1.0 / (1.0 + al * (hm ? Cm : 0.0) + ga * (hp ? Ap : 0.0));
+    const double nA = al * (hm ? Am : 0.0), nC = ga * (hp ? Cp : 0.0);
+    const double nD = D + al * (hm ? Dm : 0.0) + ga * (hp ? Dp : 0.0);
+    const double nV = V + al * (hm ? Vm : 0.0) + ga * (hp ? Vp : 0.0);
+    const double nU = U + al * (hm ? Um : 0.0) + ga * (hp ? Up : 0.0);
+    A = nA * r; C = nC * r; D = nD * r; V = nV * r; U = nU * r;
+  }
+  // x_{2l} = D - V xL - U xR. Block interface: first unknown (lane 0's even
+  // row) and last unknown (lane 63's odd row, x_127 = dO - aO x_126 - vO xL - uO xR)
+  if (lane == 0) {
+    red[6 * wave + 0] = D; red[6 * wave + 1] = V; red[6 * wave + 2] = U;
+  }
+  if (lane == 63) {
+    red[6 * wave + 3] = dO - aO * D; red[6 * wave + 4] = vO - aO * V; red[6 * wave + 5] = uO - aO * U;
+  }
+  __syncthreads();
+  // level 3: the chain F_w = df - vf L_{w-1} - uf F_{w+1}, L_w = dl - vl L_{w-1} - ul F_{w+1}
+  // (L_{-1} = F_W = 0), solved by every thread: L_{w-1} = al_w' + be_w' F_w from the left,
+  // F_w = ga_w + de_w F_{w+1}, L_w = al_w + be_w F_{w+1}
+  double xL = 0.0, xR = 0.0;
+  if (W > 1) {
+    double gam[W], del[W], alp[W], bet[W];
+    double al_prev = 0.0, be_prev = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const double df = red[6 * w], vf = red[6 * w + 1], uf = red[6 * w + 2];
+      const double dl = red[6 * w + 3], vl = red[6 * w + 4], ul = red[6 * w + 5];
+      const double r = 1.0 / (1.0 + vf * be_prev);
+      gam[w] = r * (df - vf * al_prev);
+      del[w] = -r * uf;
+      alp[w] = dl - vl * (al_prev + be_prev * gam[w]);
+      bet[w] = -vl * be_prev * del[w] - ul;
+      al_prev = alp[w];
+      be_prev = bet[w];
+    }
+    double Fn = 0.0;   // F_{w+1}
+#pragma unroll
+    for (int w = W - 1; w >= 0; w--) {
+      const double Lw = alp[w] + bet[w] * Fn;
+      const double Fw = gam[w] + del[w] * Fn;
+      if (w == wave + 1) xR = Fw;
+      if (w == wave - 1) xL = Lw;
+      Fn = Fw;
+    }
+  }
+  const double xe = D - V * xL - U * xR;
+  const double xn = __shfl(xe, (lane + 1) & 63);
+  const double xo = dO - vO * xL - uO * xR - aO * xe - cO * xn;
   if (cnt > 0) {
-    const double xf = D0[2 * tid];
-    const double xl = D0[2 * tid + 1];
-    lv[lo] = xf;
-    lv[lo + cnt - 1] = xl;
+    lv[lo] = xe;
+    lv[lo + cnt - 1] = xo;
 #pragma unroll
     for (int k = 1; k < MMAX - 1; k++)
-      if (k < cnt - 1) lv[lo + k] = dd[k] - aa[k] * xf - cc[k] * xl;
+      if (k < cnt - 1) lv[lo + k] = dd[k] - aa[k] * xe - cc[k] * xo;
   }
   __syncthreads();
 }
 
-// LDS doubles of the partitioned solve's reduced system (6 arrays of 2T)
-template <int T> constexpr int pcr_lds_doubles() { return 12 * T; }
+// LDS doubles of the partitioned solve's block interfaces (6 per wave)
+template <int T> constexpr int pcr_lds_doubles() { return 6 * (T / 64); }
 
 // one workgroup per fibre: load Vm to LDS, partitioned solve, store
 template <int T, int MMAX>
