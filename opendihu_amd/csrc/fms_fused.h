@@ -39,12 +39,12 @@ __device__ __forceinline__ void tape_store(const DevTrace &tr, long long N, int 
 // tape entry with the gates read from their HBM copy (streamed path); g == nullptr
 // stores Vm only
 template <int NS, class G>
-__device__ __forceinline__ void tape_store_streamed(const DevTrace &tr, long long N, int s, int slot0, long long i,
-                                                    double V, const G *gates) {
+__device__ __forceinline__ void tape_store_streamed(const DevTrace &tr, long long N, long long gstride, int s,
+                                                    int slot0, long long i, double V, const G *gates) {
   double *t = tr.tape + ((long long)s * (2 * NS + 1) + slot0) * N + i;
   t[0] = V;
   if (gates)
-    for (int k = 1; k < NS; k++) t[(long long)k * N] = (double)gates[(long long)(k - 1) * N + i];
+    for (int k = 1; k < NS; k++) t[(long long)k * N] = (double)gates[(long long)(k - 1) * gstride + i];
 }
 
 template <class M, class G, class E>
@@ -75,13 +75,13 @@ __device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, doub
 // and no AGPRs), so for large models the split kernels are the faster exact
 // schedule and the autotuner picks them.
 template <class M, class G, class E>
-__device__ __noinline__ double streamed_half(G *__restrict__ gates, long long N, long long node, bool store, double V,
+__device__ __noinline__ double streamed_half(G *__restrict__ gates, long long gstride, long long node, bool store, double V,
                                              const double *__restrict__ fparams, int stim_mode, int stim_param,
                                              double amplitude, bool inset, const unsigned char *flags, double t0,
                                              double dt, int n0D, double vstim) {
   G g[M::NS - 1];
 #pragma unroll
-  for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + node];
+  for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * gstride + node];
   double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
   for (int k = 0; k < M::NP; k++) p[k] = fparams[k];
@@ -96,7 +96,7 @@ __device__ __noinline__ double streamed_half(G *__restrict__ gates, long long N,
   }
   if (store) {
 #pragma unroll
-    for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + node] = g[k];
+    for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * gstride + node] = g[k];
   }
   return V;
 }
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   extern __shared__ double sh[];
   const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x;
-  const long long N = fb.n_total;
+  const long long N = fb.n_total, GS = fb.gate_stride;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
   const double *h = fb.elem_len + off - f;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
       // lanes past the fibre's end shadow its last node (computed, never stored)
       const int i = tid + q * T < n ? tid + q * T : n - 1;
 #pragma unroll
-      for (int k = 0; k < M::NS - 1; k++) greg[q][k] = gates[(long long)k * N + off + i];
+      for (int k = 0; k < M::NS - 1; k++) greg[q][k] = gates[(long long)k * GS + off + i];
       vprev[q] = 0;
       sfirst[q] = 0;
       scount[q] = 0;
@@ -180,10 +180,10 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         const bool valid = base + tid < n;
         const int i = valid ? base + tid : n - 1;
         double V = lv[i];
-        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, 0, off + i, V, gates);
-        V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, GS, 0, 0, off + i, V, gates);
+        V = streamed_half<M, G, E>(gates, GS, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
                                    sflags, t0, dt0, n0D, vstim);
-        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, 0, M::NS, off + i, V, gates);
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, GS, 0, M::NS, off + i, V, gates);
         if (valid) lv[i] = V;
       }
     }
@@ -253,8 +253,8 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         const bool valid = base + tid < n;
         const int i = valid ? base + tid : n - 1;
         double V = lv[i];
-        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s, 2 * M::NS, off + i, V, (const G *)nullptr);
-        V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
+        if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, GS, s, 2 * M::NS, off + i, V, (const G *)nullptr);
+        V = streamed_half<M, G, E>(gates, GS, off + i, valid, V, fparams, st.mode, st.stim_param, amp, i >= sb && i < se,
                                    sflags, t_mid, dt0, n0D, vstim);
         if (tr.sensor_slot && valid) {
           const int slot = tr.sensor_slot[off + i];
@@ -275,10 +275,10 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
           const bool valid = base + tid < n;
           const int i = valid ? base + tid : n - 1;
           double V = lv[i];
-          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, 0, off + i, V, gates);
-          V = streamed_half<M, G, E>(gates, N, off + i, valid, V, fparams, st.mode, st.stim_param, amp,
+          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, GS, s + 1, 0, off + i, V, gates);
+          V = streamed_half<M, G, E>(gates, GS, off + i, valid, V, fparams, st.mode, st.stim_param, amp,
                                      i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D, vstim);
-          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, s + 1, M::NS, off + i, V, gates);
+          if (tr.tape && valid) tape_store_streamed<M::NS>(tr, N, GS, s + 1, M::NS, off + i, V, gates);
           if (valid) lv[i] = V;
         }
       }
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
       const int i = tid + q * T;
       if (i < n) {
 #pragma unroll
-        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + off + i] = greg[q][k];
+        for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * GS + off + i] = greg[q][k];
         if (spk.enabled) {
           spk.vm_prev[off + i] = vprev[q];
           spk.first_step[off + i] = sfirst[q];

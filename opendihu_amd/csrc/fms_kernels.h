@@ -6,7 +6,9 @@
 //     ->  reaction half (Heun, t + dt/2)
 // Storage is state-major (SoA): Vm is one FP64 array over all nodes of all
 // fibres (fibres concatenated), the remaining NS-1 states are NS-1 arrays of
-// the gate type G (double, or float for the mixed-precision form).
+// the gate type G (double, or float for the mixed-precision form), gate k at
+// gates + k * gate_stride (the node count rounded up to 64, so that every
+// array starts on a 512-byte boundary).
 //
 // Kernels here:
 //   k_stim_flags      one thread per fibre: stimulation state machine per Heun step
@@ -34,6 +36,7 @@ namespace fms {
 struct DevFibers {
   int n_fibers;
   long long n_total;
+  long long gate_stride;     // distance between two gate arrays: n_total rounded up to 64 nodes
   const int *fiber_off;      // [F+1] first node of each fibre
   const int *node_fiber;     // [N] fibre of each node
   const double *elem_len;    // [N-F] element lengths, fibre f starts at fiber_off[f]-f
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   double V = vm[i];
   G g[M::NS - 1];
 #pragma unroll
-  for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
+  for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * fb.gate_stride + i];
   const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   double alg[ALG ? M::NA : 1] = {};
   const int nj = PAIR ? 2 * n0D : n0D;
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   if (valid) {
     vm[i] = V;
 #pragma unroll
-    for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + i] = g[k];
+    for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * fb.gate_stride + i] = g[k];
     if constexpr (ALG) {
       for (int k = 0; k < n_alg_transfer; k++) {
         const int a = alg_idx[k];
@@ -364,8 +367,9 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
 
 // ----------------------------------------------------------------------------
 // reaction with one workgroup per fibre (large models): the fibre's nodes in
-// passes of T lanes (lanes past the end shadow the last node and store
-// nothing), the fibre's flags and parameters uniform over the workgroup.
+// passes of T lanes aligned to 32-node segments of the arrays (lanes outside
+// the fibre shadow its first or last node and store nothing), the fibre's
+// flags and parameters uniform over the workgroup.
 // PAIR as for k_reaction.
 // ----------------------------------------------------------------------------
 template <class M, class G, class E, int T, bool PAIR = false>
@@ -384,9 +388,16 @@ __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, 
   const double amp = st.mode == 2 ? st.amplitude[f] : 0.0;
   const int nj = PAIR ? 2 * n0D : n0D;
   const int spike_step = PAIR ? (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1) : 0;
-  for (int base = 0; base < n; base += T) {
-    const bool valid = base + (int)threadIdx.x < n;
-    const int li = valid ? base + (int)threadIdx.x : n - 1;
+  // passes start on a 32-node boundary of the concatenated arrays (a 256-byte
+  // FP64 or 128-byte FP32 segment): lanes before the fibre's first node shadow
+  // it, lanes past its end shadow its last node, and shadows store nothing. A
+  // fibre's node count plus its offset within the segment rarely needs one more
+  // pass (1481 nodes: 24 passes either way).
+  const int lead = off & 31;
+  for (int base = -lead; base < n; base += T) {
+    const int lr = base + (int)threadIdx.x;
+    const bool valid = lr >= 0 && lr < n;
+    const int li = lr < 0 ? 0 : (lr < n ? lr : n - 1);
     const long long i = off + li;
     const bool inset = li >= sb && li < se;
     double p[M::NP > 0 ? M::NP : 1];
@@ -396,7 +407,7 @@ __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, 
     double V = vm[i];
     G g[M::NS - 1];
 #pragma unroll
-    for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * N + i];
+    for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * fb.gate_stride + i];
     for (int j = 0; j < nj; j++) {
       const bool second = PAIR && j >= n0D;
       const int jj = second ? j - n0D : j;
@@ -415,7 +426,7 @@ __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, 
     if (valid) {
       vm[i] = V;
 #pragma unroll
-      for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * N + i] = g[k];
+      for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * fb.gate_stride + i] = g[k];
     }
   }
 }

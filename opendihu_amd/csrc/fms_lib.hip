@@ -99,6 +99,7 @@ struct FmsSolver {
   // configuration
   int F = 0;
   long long N = 0;
+  long long GS = 0;   // gate stride: N rounded up to 64 nodes (every gate array 512-byte aligned)
   int scheme = FMS_IMPLICIT_EULER, precision = FMS_FP64, exp_mode = FMS_EXP_APPROXIMATE;
   int thomas = FMS_THOMAS_PARTITIONED, kernels = FMS_KERNELS_SPLIT;
   double vstim = 20.0;
@@ -114,7 +115,7 @@ struct FmsSolver {
   int *d_off = nullptr, *d_node_fiber = nullptr, *d_sb = nullptr, *d_se = nullptr, *d_order = nullptr;
   double *d_h = nullptr, *d_pf = nullptr, *d_params = nullptr;
   double *d_vm = nullptr;
-  void *d_gates = nullptr;    // double or float [(NS-1)*N]
+  void *d_gates = nullptr;    // double or float [(NS-1)*GS]
   double *d_cA = nullptr, *d_dA = nullptr;  // serial Thomas scratch (split kernels)
   unsigned char *d_flags = nullptr;
   int flags_cap = 0;
@@ -196,6 +197,7 @@ struct FmsSolver {
     memset(&f, 0, sizeof(f));
     f.n_fibers = F;
     f.n_total = N;
+    f.gate_stride = GS;
     f.fiber_off = d_off;
     f.node_fiber = d_node_fiber;
     f.elem_len = d_h;
@@ -881,6 +883,7 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
     s->h_off[f + 1] = (int)total;
   }
   s->N = total;
+  s->GS = (total + 63) / 64 * 64;
   if (s->nmax > 4096 && (s->thomas == FMS_THOMAS_PARTITIONED || s->kernels == FMS_KERNELS_FUSED)) {
     delete s;
     return fail(FMS_ERR_UNSUPPORTED, "fibres longer than 4096 nodes need thomas=SERIAL and kernels=SPLIT");
@@ -925,7 +928,7 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
   chk(dalloc(&s->d_vm, (size_t)N));
   chk(dalloc(&s->d_hasbeen, (size_t)F));
   chk(dalloc(&s->d_cursor, 4));
-  chk(hipMalloc(&s->d_gates, (size_t)(Model::NS - 1) * N * gate_bytes(s) + 8) == hipSuccess ? 0 : FMS_ERR_HIP);
+  chk(hipMalloc(&s->d_gates, (size_t)(Model::NS - 1) * s->GS * gate_bytes(s) + 8) == hipSuccess ? 0 : FMS_ERR_HIP);
   if (s->thomas == FMS_THOMAS_SERIAL && s->kernels == FMS_KERNELS_SPLIT) {
     chk(dalloc(&s->d_cA, (size_t)N));
     chk(dalloc(&s->d_dA, (size_t)N));
@@ -959,13 +962,14 @@ int fms_create(const FmsConfig *c, FmsSolver **out) {
     std::vector<double> v((size_t)N, y0[0]);
     chk(h2d(s->d_vm, v.data(), (size_t)N, s->stream));
     if (s->precision == FMS_FP64) {
-      std::vector<double> g((size_t)(Model::NS - 1) * N);
-      for (int k = 1; k < Model::NS; k++) std::fill(g.begin() + (size_t)(k - 1) * N, g.begin() + (size_t)k * N, y0[k]);
+      std::vector<double> g((size_t)(Model::NS - 1) * s->GS, 0.0);
+      for (int k = 1; k < Model::NS; k++)
+        std::fill(g.begin() + (size_t)(k - 1) * s->GS, g.begin() + (size_t)(k - 1) * s->GS + N, y0[k]);
       chk(h2d((double *)s->d_gates, g.data(), g.size(), s->stream));
     } else {
-      std::vector<float> g((size_t)(Model::NS - 1) * N);
+      std::vector<float> g((size_t)(Model::NS - 1) * s->GS, 0.0f);
       for (int k = 1; k < Model::NS; k++)
-        std::fill(g.begin() + (size_t)(k - 1) * N, g.begin() + (size_t)k * N, (float)y0[k]);
+        std::fill(g.begin() + (size_t)(k - 1) * s->GS, g.begin() + (size_t)(k - 1) * s->GS + N, (float)y0[k]);
       chk(h2d((float *)s->d_gates, g.data(), g.size(), s->stream));
     }
   }
@@ -1170,11 +1174,11 @@ int fms_get_states(FmsSolver *s, int k, double *out) {
   if (k == 0) {
     HIP_TRY(hipMemcpy(out, s->d_vm, (size_t)s->N * sizeof(double), hipMemcpyDeviceToHost));
   } else if (s->precision == FMS_FP64) {
-    HIP_TRY(hipMemcpy(out, (double *)s->d_gates + (size_t)(k - 1) * s->N, (size_t)s->N * sizeof(double),
+    HIP_TRY(hipMemcpy(out, (double *)s->d_gates + (size_t)(k - 1) * s->GS, (size_t)s->N * sizeof(double),
                       hipMemcpyDeviceToHost));
   } else {
     std::vector<float> tmp((size_t)s->N);
-    HIP_TRY(hipMemcpy(tmp.data(), (float *)s->d_gates + (size_t)(k - 1) * s->N, (size_t)s->N * sizeof(float),
+    HIP_TRY(hipMemcpy(tmp.data(), (float *)s->d_gates + (size_t)(k - 1) * s->GS, (size_t)s->N * sizeof(float),
                       hipMemcpyDeviceToHost));
     for (long long i = 0; i < s->N; i++) out[i] = tmp[(size_t)i];
   }
@@ -1190,12 +1194,12 @@ int fms_set_states(FmsSolver *s, int k, const double *in) {
   if (k == 0) {
     HIP_TRY(hipMemcpy(s->d_vm, in, (size_t)s->N * sizeof(double), hipMemcpyHostToDevice));
   } else if (s->precision == FMS_FP64) {
-    HIP_TRY(hipMemcpy((double *)s->d_gates + (size_t)(k - 1) * s->N, in, (size_t)s->N * sizeof(double),
+    HIP_TRY(hipMemcpy((double *)s->d_gates + (size_t)(k - 1) * s->GS, in, (size_t)s->N * sizeof(double),
                       hipMemcpyHostToDevice));
   } else {
     std::vector<float> tmp((size_t)s->N);
     for (long long i = 0; i < s->N; i++) tmp[(size_t)i] = (float)in[i];
-    HIP_TRY(hipMemcpy((float *)s->d_gates + (size_t)(k - 1) * s->N, tmp.data(), (size_t)s->N * sizeof(float),
+    HIP_TRY(hipMemcpy((float *)s->d_gates + (size_t)(k - 1) * s->GS, tmp.data(), (size_t)s->N * sizeof(float),
                       hipMemcpyHostToDevice));
   }
   return 0;
@@ -1441,9 +1445,12 @@ template <class T> int ensure(T **p, long long &cap, long long n) {
   return 0;
 }
 
+// one checkpoint: Vm (N doubles) then the gate arrays with their stride
+size_t ckpt_doubles(const FmsSolver *s) { return (size_t)s->N + (size_t)(Model::NS - 1) * s->GS; }
+
 int copy_state(FmsSolver *s, double *dst, bool to_ckpt) {
   const size_t n = (size_t)s->N;
-  const size_t gb = (size_t)(Model::NS - 1) * n * sizeof(double);
+  const size_t gb = (size_t)(Model::NS - 1) * s->GS * sizeof(double);
   if (to_ckpt) {
     HIP_TRY(hipMemcpyAsync(dst, s->d_vm, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(dst + n, s->d_gates, gb, hipMemcpyDeviceToDevice, s->stream));
@@ -1500,7 +1507,7 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
       // the buffer was last read by the reverse of segment seg+2
       s->stream = s->stream2;
       if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess) rc = FMS_ERR_HIP;
-      if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, false);
+      if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), false);
       fms::DevTrace tr{};
       tr.tape = tape;
       const long long save = s->step_counter;
@@ -1556,7 +1563,7 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
   const int nseg = (K + Kc - 1) / Kc;
   int rc;
-  if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)nseg * Model::NS * N))) return rc;
+  if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)(nseg * ckpt_doubles(s))))) return rc;
   if (ensure_flags(s, n0D)) return FMS_ERR_HIP;
   if ((rc = upload_steps(s, steps, K))) return rc;
   s->adj_steps.assign(steps, steps + 4 * (size_t)K);
@@ -1565,7 +1572,7 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   s->adj_K = K;
   for (int seg = 0; seg < nseg; seg++) {
     const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
-    if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * Model::NS * N, true))) return rc;
+    if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), true))) return rc;
     fms::DevTrace tr{};
     tr.sensor_slot = s->d_sensor_slot;
     tr.trace = d_traces + (size_t)k0 * F * S;
@@ -1709,7 +1716,7 @@ int snapshot_take(FmsSolver *s, Snapshot &sn) {
     if (p && bytes) sn.regions.push_back({p, bytes});
   };
   add(s->d_vm, N * sizeof(double));
-  add(s->d_gates, (size_t)(Model::NS - 1) * N * gate_bytes(s));
+  add(s->d_gates, (size_t)(Model::NS - 1) * s->GS * gate_bytes(s));
   add(s->d_last, F * sizeof(double));
   add(s->d_curjit, F * sizeof(double));
   add(s->d_jidx, F * sizeof(int));
