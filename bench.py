@@ -48,12 +48,22 @@ FP64_PEAK_TFLOPS = 78.6
 SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 
 # HBM bytes per node and launch of the dominant kernel, measured with rocprofv3
-# (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB device copy), keyed by
-# (workload, kernel class, threads per fibre of the per-fibre reaction):
-# profiles/k_reaction_fibre_pair_shorten56_1024x1481.md,
-# profiles/k_reaction_fibre_pair_tk2014_1024x1481.md
-MEASURED_TRAFFIC_PER_NODE_LAUNCH = {("c2", "reaction_pair", 64): 1025.0, ("c2", "reaction_pair", 256): 928.4,
-                                    ("c2", "reaction", 256): 925.8, ("tk2014", "reaction_pair", 64): 1042.6}
+# (FETCH_SIZE and WRITE_SIZE scaled by what they read for a 1 GiB device copy),
+# one record per (workload, kernel class, threads per fibre of the per-fibre
+# reaction) in profiles/traffic.json, each naming the profile summary it comes from
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.json")
+
+
+def measured_traffic(workload: str, kclass: str, threads: int):
+    """(bytes per node and launch, source) of the dominant kernel, or (None, None)"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for r in json.load(f)["records"]:
+                if (r["workload"], r["kernel_class"], r["threads"]) == (workload, kclass, threads):
+                    return r["bytes_per_node_launch"], r["source"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None, None
 
 
 def bytes_per_node_step(n_states: int, precision: str) -> int:
@@ -82,7 +92,10 @@ def make_workload(name: str, precision: str, kernels: str):
         g.set_firing_table_stimulation(**stim)
         desc = dict(workload="shorten56_1024x1481_IE_dt2.5e-5", model=SHORTEN, fibers=1024, nodes_per_fiber=1481,
                     scheme="ImplicitEuler", dt_split_ms=2.5e-5, motor_units=10, firing_hz=20,
-                    stimulation="firing table, Vm clamp (reference vc semantics, W=4)")
+                    motor_units_firing_in_5ms=int(stim["firing_events"].any(axis=0).sum()),
+                    stimulation="firing table, Vm clamp (reference vc semantics, W=4)",
+                    work="every node's reaction and diffusion every step (onlyComputeIfHasBeenStimulated and "
+                         "the equilibrium skip off)")
         return g, mesh, stim, desc, 2.5e-5, "IE", SHORTEN
     if name == "tk2014":
         from opendihu_amd.build import TK2014_LIBRARY
@@ -92,7 +105,8 @@ def make_workload(name: str, precision: str, kernels: str):
         g.set_firing_table_stimulation(**stim)
         desc = dict(workload="tk2014_57_1024x1481_IE_dt2.5e-5", model="tk2014 (57 states, 71 algebraics)",
                     fibers=1024, nodes_per_fiber=1481, scheme="ImplicitEuler", dt_split_ms=2.5e-5, motor_units=10,
-                    firing_hz=20, stimulation="firing table, Vm clamp (reference vc semantics, W=4)")
+                    firing_hz=20, motor_units_firing_in_5ms=int(stim["firing_events"].any(axis=0).sum()),
+                    stimulation="firing table, Vm clamp (reference vc semantics, W=4)")
         return g, mesh, stim, desc, 2.5e-5, "IE", "tk2014"
     if name == "c3":
         mesh, stim = uneven(4096, seed=3)
@@ -129,36 +143,65 @@ def time_gpu(g, dt, steps, warmup):
     return wall, ev_ms, launches
 
 
-def cpu_baseline(mesh, stim, scheme, model, dt, budget_s=12.0):
-    """FP64 oracle on the host cores over a bounded sample of the workload."""
-    from oracle_binding import OracleBatch
-    threads = min(os.cpu_count() or 1, 16)
-    o = OracleBatch(**mesh, scheme=scheme, model=model, threads=threads)
-    if "firing_events" in stim:
-        o.set_firing_table_stimulation(**stim)
-    else:
-        o.set_current_stimulation(**stim)
-    n_total = o.n_total
-    t0 = time.perf_counter()
-    o.advance(0.0, dt, 1)
-    probe = time.perf_counter() - t0
-    steps = max(1, min(2000, int(budget_s / max(probe, 1e-6))))
-    t0 = time.perf_counter()
-    o.advance(dt, dt, steps)
-    el = time.perf_counter() - t0
-    cpu = "unknown"
+def host_cpu() -> dict:
+    """the host's CPU model, logical CPUs, the CPUs this process may run on, and AVX-512"""
+    model, avx512 = "unknown", False
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("flags") and " avx512f" in line:
+                    avx512 = True
     except OSError:
         pass
-    return dict(value=n_total * steps / el, unit="fiber-node-steps/s", cores=threads, kind="port",
-                sample=f"all {len(mesh['n_nodes'])} fibres ({n_total} nodes) x {steps} split steps "
-                       f"({el:.1f} s, FP64 oracle, four nodes per AVX2 vector operation, {threads} OpenMP "
-                       f"threads on {cpu})")
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return dict(model=model, logical_cpus=os.cpu_count(), allowed_cpus=allowed, avx512=avx512)
+
+
+def cpu_baseline(mesh, stim, scheme, model, dt, budget_s=10.0):
+    """FP64 oracle (the reference's vc-path restatement) on the host over a
+    bounded sample of the workload: W = 4 nodes per vector operation (the
+    reference's Vc double_v on AVX2) and, on an AVX-512 host, W = 8; value is the
+    faster. Threads: every CPU this process may run on, capped by OMP_NUM_THREADS
+    where that is set (the CPU share of one GPU on a shared host)."""
+    from oracle_binding import OracleBatch, ORACLE_BUILD
+    cpu = host_cpu()
+    threads = cpu["allowed_cpus"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
+    variants = [("", 4)]
+    if cpu["avx512"] and os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{model}_w8.so")):
+        variants.append(("_w8", 8))
+    runs = {}
+    for suffix, width in variants:
+        o = OracleBatch(**mesh, scheme=scheme, model=model + suffix, threads=threads)
+        if "firing_events" in stim:
+            o.set_firing_table_stimulation(**stim)
+        else:
+            o.set_current_stimulation(**stim)
+        n_total = o.n_total
+        t0 = time.perf_counter()
+        o.advance(0.0, dt, 1)
+        probe = time.perf_counter() - t0
+        steps = max(1, min(2000, int(budget_s / len(variants) / max(probe, 1e-6))))
+        t0 = time.perf_counter()
+        o.advance(dt, dt, steps)
+        el = time.perf_counter() - t0
+        runs[width] = dict(value=n_total * steps / el, steps=steps, seconds=el)
+        del o
+    best = max(runs, key=lambda w: runs[w]["value"])
+    n_total = int(np.sum(mesh["n_nodes"]))
+    return dict(value=runs[best]["value"], unit="fiber-node-steps/s", cores=threads, kind="port",
+                lanes=best, by_lanes={f"W{w}": r["value"] for w, r in runs.items()}, host=cpu,
+                sample=f"all {len(mesh['n_nodes'])} fibres ({n_total} nodes) x "
+                       + ", ".join(f"{r['steps']} split steps ({r['seconds']:.1f} s, W={w})" for w, r in runs.items())
+                       + f"; FP64 oracle, W nodes per vector operation as the reference's Vc path, {threads} "
+                       f"OpenMP threads on {cpu['model']} ({cpu['allowed_cpus']} CPUs allowed, "
+                       f"{cpu['logical_cpus']} logical)")
 
 
 def _spike_shift(first_a, first_b, n_steps):
@@ -239,8 +282,10 @@ def bench_c4():
     dev = torch.device("cuda")
     target = prob.forward_tensor(torch.tensor(theta_true, device=dev))
     th0 = torch.tensor(theta0, device=dev)
-    # warm-up of every path (code objects, buffers)
+    # warm-up of every path (code objects, buffers); the adjoint schedule (segment
+    # recompute beside the reverse sweep or before it) timed and chosen here
     prob.backward_tensor(prob.forward_tensor(th0) - target)
+    tuned = prob.tune_backward(theta0, (prob.forward(theta0) - target.cpu().numpy()))
     # plain forward solve (no traces, no checkpoints) at theta0
     prob.set_theta(theta0)
     for k in range(b.n_states):
@@ -269,6 +314,7 @@ def bench_c4():
     return dict(workload="hh_adjoint_256x1000_CN_dt1e-3_5ms_32sensors_ckpt100", steps=n_steps,
                 plain_forward_s=t_plain, recording_forward_s=t_fwd, backward_s=t_bwd,
                 gradient_cost_vs_forward=(t_fwd + t_bwd) / t_plain, backward_vs_forward=t_bwd / t_plain,
+                adjoint_schedule=tuned,
                 directional_derivative=dict(adjoint=ad, central_fd=fd, h=h, rel_err=abs(ad - fd) / abs(fd),
                                             central_fd_2h=fd2, richardson=fdr, rel_err_richardson=abs(ad - fdr) / abs(fdr)),
                 note="wall-clock per call; traces and their adjoints stay in device memory (torch tensors)")
@@ -417,7 +463,7 @@ def main(argv=None):
     achieved_tflops = f_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e12
     achieved_gbs = b_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e9
     rthreads = (schedule or {}).get("reaction_threads", 256)
-    traffic_per_node = MEASURED_TRAFFIC_PER_NODE_LAUNCH.get((args.workload, kclass, rthreads))
+    traffic_per_node, traffic_source = measured_traffic(args.workload, kclass, rthreads)
     kernel_times = {k: dict(total_ms=v[0], launches=v[1]) for k, v in times.items()}
     g.close()
 
@@ -475,6 +521,7 @@ def main(argv=None):
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
             "traffic": traffic_per_node * n_total if traffic_per_node else None,
+            "traffic_source": traffic_source,
             "ops_per_node_and_launch_unit": f_unit,
             "nodes_x_units_per_launch": units_per_launch,
             "mean_launch_ms": mean_launch_ms,

@@ -209,6 +209,22 @@ int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [
 #define FMS_TUNE_PAIR_HALVES 8        /* split kernels: 1 (default) runs the second reaction half of step q and the
                                          first of step q+1 (no solve between them) as one launch that reads and
                                          writes the states once; 0: one launch per half. Same arithmetic. */
+/* adjoint backward (fms_adjoint_backward): each checkpoint segment is recomputed
+   (recording its tape) and then reversed. FMS_ADJOINT_OVERLAP: the recompute
+   of the next segment runs on a second stream beside the reverse of this one;
+   _OVERLAP_PRIORITY: the same, the reverse sweep's waves with raised issue
+   priority; _SERIAL (default): recompute and reverse one after the other on the
+   solver's stream. Same arithmetic, same gradients. */
+#define FMS_TUNE_ADJOINT 10
+#define FMS_ADJOINT_OVERLAP 0
+#define FMS_ADJOINT_OVERLAP_PRIORITY 1
+#define FMS_ADJOINT_SERIAL 2
+#define FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY 3   /* beside it, the recompute's waves with raised priority */
+/* threads per fibre workgroup of the adjoint's reverse sweep: 256, 512
+   (default) or 1024 (fewer nodes per thread, more waves per SIMD). The prefactor
+   gradient is summed per wave and then over the waves, so its last bits depend
+   on this number. */
+#define FMS_TUNE_ADJOINT_THREADS 11
 #define FMS_TUNE_REACTION_THREADS 9   /* split kernels, models of > 8 states, reaction per fibre: threads per fibre
                                          workgroup, 256 (default) or 64 (one wave: four fibres share a CU and their
                                          loads do not coincide). Same arithmetic. */

@@ -21,7 +21,10 @@ FMS_STIM_NONE, FMS_STIM_FIRING_TABLE, FMS_STIM_CURRENT = 0, 1, 2
 FMS_OK, FMS_ERR_ARGUMENT, FMS_ERR_HIP, FMS_ERR_STATE, FMS_ERR_UNSUPPORTED, FMS_ERR_INTERNAL = 0, -1, -2, -3, -4, -5
 FMS_TUNE_STEPS_PER_LAUNCH, FMS_TUNE_RESIDENT_GATES, FMS_TUNE_ONLY_IF_STIMULATED = 1, 2, 3
 FMS_TUNE_THREADS, FMS_TUNE_GRAPH_STEPS, FMS_TUNE_REACTION_PER_FIBRE, FMS_TUNE_PROFILE_KERNELS = 4, 5, 6, 7
-FMS_TUNE_PAIR_HALVES, FMS_TUNE_REACTION_THREADS = 8, 9
+FMS_TUNE_PAIR_HALVES, FMS_TUNE_REACTION_THREADS, FMS_TUNE_ADJOINT = 8, 9, 10
+FMS_ADJOINT_OVERLAP, FMS_ADJOINT_OVERLAP_PRIORITY, FMS_ADJOINT_SERIAL = 0, 1, 2
+FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY = 3
+FMS_TUNE_ADJOINT_THREADS = 11
 FMS_KERNEL_CLASSES = ("reaction", "diffusion", "fused", "other", "reaction_pair")
 
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -97,6 +97,27 @@ class AdjointProblem:
                 "fms_adjoint_backward_device")
         return torch.cat([gp, (gpf / self.am_cm)[:, None]], dim=1)
 
+    def tune_backward(self, theta, dtraces) -> dict:
+        """time fms_adjoint_backward under each adjoint schedule (segment
+        recompute before the reverse sweep, or beside it with equal priority,
+        the sweep's or the recompute's priority raised) for one forward at theta
+        and keep the fastest; the gradients are the same under every schedule"""
+        import time
+        from . import _lib
+        times = {}
+        self.forward(theta)
+        for mode in (_lib.FMS_ADJOINT_SERIAL, _lib.FMS_ADJOINT_OVERLAP, _lib.FMS_ADJOINT_OVERLAP_PRIORITY,
+                     _lib.FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY):
+            self.batch.set_tuning(adjoint=mode)
+            self.backward(dtraces)   # untimed: buffers, code objects
+            t0 = time.perf_counter()
+            self.backward(dtraces)
+            times[mode] = time.perf_counter() - t0
+        best = min(times, key=times.get)
+        self.batch.set_tuning(adjoint=best)
+        names = {0: "overlap", 1: "overlap_priority", 2: "serial", 3: "overlap_recompute_priority"}
+        return dict(schedule=names[best], seconds={names[k]: v for k, v in times.items()})
+
     def backward(self, dtraces) -> np.ndarray:
         """dL/dtheta for the last forward; theta columns (g_Na, g_K, g_L, sigma)"""
         d = np.ascontiguousarray(dtraces, np.float64)

@@ -246,11 +246,11 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
                                                  const double *__restrict__ steps, int kc, int n0D, int ie,
                                                  const int *__restrict__ sensor_slot,
                                                  const double *__restrict__ dtrace, int n_sensors, double *lam,
-                                                 double *gacc, double *pf_bar) {
+                                                 double *gacc, double *pf_bar, int raise_priority) {
   extern __shared__ double sh[];
-  // the reverse sweep is the critical path while the next segment's recompute
-  // runs beside it on the same CUs: its waves get issue priority
-  __builtin_amdgcn_s_setprio(2);
+  // with raise_priority the reverse sweep's waves win issue over the next
+  // segment's recompute that runs beside it on the same CUs
+  if (raise_priority) __builtin_amdgcn_s_setprio(2);
   const int f = blockIdx.x;
   const int tid = threadIdx.x;
   const long long N = fb.n_total;
@@ -262,6 +262,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   double *lw = sh + pcr_lds_doubles<T>();    // n: lambda_V, then w
   double *lv = lw + n;       // n: the diffusion's V (IE: V^b, CN: V^a + V^b), for the neighbours
   double p[4];
+#pragma unroll
   for (int k = 0; k < 4; k++) p[k] = fb.params[(long long)f * 4 + k];
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   double l[NPT][4], g[NPT][3];

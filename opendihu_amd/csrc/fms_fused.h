@@ -26,6 +26,7 @@ struct DevTrace {
   int n_sensors;
   double *tape;             // [K][2*NS+1][N]: y before half 1 (NS), y after half 1 (NS: Vm
                             // before the diffusion and the gates), Vm after it (1); nullptr: none
+  int raise_priority;       // the waves win issue over a kernel running beside them
 };
 
 template <int NS, class G>
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
                                              int n0D, double vstim, int ie, int serial, const int *__restrict__ order,
                                              DevSpikes spk, DevTrace tr) {
   extern __shared__ double sh[];
+  if (tr.raise_priority) __builtin_amdgcn_s_setprio(2);
   const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x;
   const long long N = fb.n_total, GS = fb.gate_stride;

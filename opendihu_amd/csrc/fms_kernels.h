@@ -444,35 +444,35 @@ FMS_HD void assemble_row(int i, int n, const double *u, const double *h, double 
   if (i > 0) {
     const double u_previous = u[i - 1];
     const double h_left = h[i - 1];
-    const double k_left = 1. / h_left * (1) * prefactor;
-    const double m_left = h_left * 1. / 6;
-    const double k_right = 1. / h_left * (-1) * prefactor;
-    const double m_right = h_left * 1. / 3;
+    const double k_left = div_normal(1., h_left) * (1) * prefactor;
+    const double m_left = div_normal(h_left * 1., 6);
+    const double k_right = div_normal(1., h_left) * (-1) * prefactor;
+    const double m_right = div_normal(h_left * 1., 3);
     if (ie) {
-      a = (k_left - 1 / dt * m_left);
-      b += (k_right - 1 / dt * m_right);
-      d += (-1 / dt * m_left) * u_previous + (-1 / dt * m_right) * u_center;
+      a = (k_left - div_normal(1, dt) * m_left);
+      b += (k_right - div_normal(1, dt) * m_right);
+      d += (-div_normal(1, dt) * m_left) * u_previous + (-div_normal(1, dt) * m_right) * u_center;
     } else {
-      a = (k_left / 2. - 1 / dt * m_left);
-      b += (k_right / 2. - 1 / dt * m_right);
-      d += (-k_left / 2. - 1 / dt * m_left) * u_previous + (-k_right / 2. - 1 / dt * m_right) * u_center;
+      a = (k_left / 2. - div_normal(1, dt) * m_left);
+      b += (k_right / 2. - div_normal(1, dt) * m_right);
+      d += (-k_left / 2. - div_normal(1, dt) * m_left) * u_previous + (-k_right / 2. - div_normal(1, dt) * m_right) * u_center;
     }
   }
   if (i < n - 1) {
     const double u_next = u[i + 1];
     const double h_right = h[i];
-    const double k_right = 1. / h_right * (1) * prefactor;
-    const double m_right = h_right * 1. / 6;
-    const double k_left = 1. / h_right * (-1) * prefactor;
-    const double m_left = h_right * 1. / 3;
+    const double k_right = div_normal(1., h_right) * (1) * prefactor;
+    const double m_right = div_normal(h_right * 1., 6);
+    const double k_left = div_normal(1., h_right) * (-1) * prefactor;
+    const double m_left = div_normal(h_right * 1., 3);
     if (ie) {
-      c = (k_right - 1 / dt * m_right);
-      b += (k_left - 1 / dt * m_left);
-      d += (-1 / dt * m_left) * u_center + (-1 / dt * m_right) * u_next;
+      c = (k_right - div_normal(1, dt) * m_right);
+      b += (k_left - div_normal(1, dt) * m_left);
+      d += (-div_normal(1, dt) * m_left) * u_center + (-div_normal(1, dt) * m_right) * u_next;
     } else {
-      c = (k_right / 2. - 1 / dt * m_right);
-      b += (k_left / 2. - 1 / dt * m_left);
-      d += (-k_left / 2. - 1 / dt * m_left) * u_center + (-k_right / 2. - 1 / dt * m_right) * u_next;
+      c = (k_right / 2. - div_normal(1, dt) * m_right);
+      b += (k_left / 2. - div_normal(1, dt) * m_left);
+      d += (-k_left / 2. - div_normal(1, dt) * m_left) * u_center + (-k_right / 2. - div_normal(1, dt) * m_right) * u_next;
     }
   }
 }
@@ -485,11 +485,11 @@ FMS_HD void thomas_serial_fiber(double *u, int n, const double *h, double prefac
     double a, b, c, d;
     assemble_row(i, n, u, h, prefactor, dt, ie, a, b, c, d);
     if (i == 0) {
-      cA[i] = c / b;
-      dA[i] = d / b;
+      cA[i] = div_normal(c, b);
+      dA[i] = div_normal(d, b);
     } else {
-      if (i != n - 1) cA[i] = c / (b - cA[i - 1] * a);
-      dA[i] = (d - dA[i - 1] * a) / (b - cA[i - 1] * a);
+      if (i != n - 1) cA[i] = div_normal(c, b - cA[i - 1] * a);
+      dA[i] = div_normal(d - dA[i - 1] * a, b - cA[i - 1] * a);
     }
   }
   u[n - 1] = dA[n - 1];
@@ -560,11 +560,11 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
         assemble_row(lo + k, n, lv, h, prefactor, dt, ie, a, b, c, d);
         if (GIVEN_RHS) d = lv[lo + k];
         if (k < 2) {
-          aa[k] = a / b;
-          cc[k] = c / b;
-          dd[k] = d / b;
+          aa[k] = div_normal(a, b);
+          cc[k] = div_normal(c, b);
+          dd[k] = div_normal(d, b);
         } else {
-          const double r = 1.0 / (b - a * cc[k - 1]);
+          const double r = div_normal(1.0, b - a * cc[k - 1]);
           dd[k] = r * (d - a * dd[k - 1]);
           aa[k] = -r * a * aa[k - 1];
           cc[k] = r * c;
@@ -580,7 +580,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
       }
     }
     if (cnt >= 3) {
-      const double r = 1.0 / (1.0 - cc[0] * aa[1]);
+      const double r = div_normal(1.0, 1.0 - cc[0] * aa[1]);
       dd[0] = r * (dd[0] - cc[0] * dd[1]);
       aa[0] = r * aa[0];
       cc[0] = -r * cc[0] * cc[1];
@@ -610,7 +610,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     const double al = -aE, ga = -cE;
     A = al * pa;
     C = ga * cO;
-    const double r = 1.0 / (1.0 + al * pc + ga * aO);
+    const double r = div_normal(1.0, 1.0 + al * pc + ga * aO);
     D = r * (dE + al * pd + ga * dO);
     V = r * (vE + al * pv + ga * vO);
     U = r * (uE + al * pu + ga * uO);
@@ -625,7 +625,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     const double Am = __shfl(A, im), Cm = __shfl(C, im), Dm = __shfl(D, im), Vm = __shfl(V, im), Um = __shfl(U, im);
     const double Ap = __shfl(A, ip), Cp = __shfl(C, ip), Dp = __shfl(D, ip), Vp = __shfl(V, ip), Up = __shfl(U, ip);
     const double al = hm ? -A : 0.0, ga = hp ? -C : 0.0;
-    const double r = 1.0 / (1.0 + al * (hm ? Cm : 0.0) + ga * (hp ? Ap : 0.0));
+    const double r = div_normal(1.0, 1.0 + al * (hm ? Cm : 0.0) + ga * (hp ? Ap : 0.0));
     const double nA = al * (hm ? Am : 0.0), nC = ga * (hp ? Cp : 0.0);
     const double nD = D + al * (hm ? Dm : 0.0) + ga * (hp ? Dp : 0.0);
     const double nV = V + al * (hm ? Vm : 0.0) + ga * (hp ? Vp : 0.0);
@@ -652,7 +652,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     for (int w = 0; w < W; w++) {
       const double df = red[6 * w], vf = red[6 * w + 1], uf = red[6 * w + 2];
       const double dl = red[6 * w + 3], vl = red[6 * w + 4], ul = red[6 * w + 5];
-      const double r = 1.0 / (1.0 + vf * be_prev);
+      const double r = div_normal(1.0, 1.0 + vf * be_prev);
       gam[w] = r * (df - vf * al_prev);
       del[w] = -r * uf;
       alp[w] = dl - vl * (al_prev + be_prev * gam[w]);

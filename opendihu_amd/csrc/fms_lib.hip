@@ -157,6 +157,8 @@ struct FmsSolver {
   int tune_reaction_fibre = 1;   // split kernels, large models: one workgroup per fibre for the reaction
   int tune_pair = 1;   // split kernels: the two reaction halves between two solves in one launch
   int tune_reaction_threads = 256;   // split kernels, per-fibre reaction: threads per fibre workgroup (64 or 256)
+  int tune_adjoint = FMS_ADJOINT_SERIAL;    // adjoint backward: recompute beside the reverse sweep or before it
+  int tune_adjoint_threads = 512;           // threads per fibre workgroup of the reverse sweep (256, 512, 1024)
 
   // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): HIP events
   // around every launch on the solver's stream, waited for one by one
@@ -412,9 +414,18 @@ int launch_thomas(FmsSolver *s, int q) {
     hipLaunchKernelGGL(fms::k_thomas_serial, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
                        s->fibers(), s->d_vm, s->d_cA, s->d_dA, step_ref(s, q), ie);
   } else {
+    // rows per chunk: at most ceil(nmax / 256) (>= 3 for fibres under 512 nodes);
+    // the smallest register array that holds them
     const size_t lds = (size_t)(fms::pcr_lds_doubles<PART_T>() + s->nmax) * sizeof(double);
-    hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 16>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
-                       s->d_vm, step_ref(s, q), ie);
+    if (s->nmax <= 1024)
+      hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 4>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
+                         s->d_vm, step_ref(s, q), ie);
+    else if (s->nmax <= 2048)
+      hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 8>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
+                         s->d_vm, step_ref(s, q), ie);
+    else
+      hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 16>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
+                         s->d_vm, step_ref(s, q), ie);
   }
   HIP_TRY(hipGetLastError());
   s->n_launch_thomas++;
@@ -1324,6 +1335,19 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "paired reaction halves is 0 or 1");
       s->tune_pair = value;
       return 0;
+    case FMS_TUNE_ADJOINT:
+      if (value < FMS_ADJOINT_OVERLAP || value > FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY)
+        return fail(FMS_ERR_ARGUMENT, "adjoint schedule is FMS_ADJOINT_OVERLAP, _OVERLAP_PRIORITY, _SERIAL or "
+                                      "_OVERLAP_RECOMPUTE_PRIORITY");
+      s->tune_adjoint = value;
+      return 0;
+    case FMS_TUNE_ADJOINT_THREADS:
+      if (value != 256 && value != 512 && value != 1024)
+        return fail(FMS_ERR_ARGUMENT, "adjoint threads per fibre are 256, 512 or 1024");
+      if (value > 256 && s->nmax > 2048 * value / 1024)
+        return fail(FMS_ERR_UNSUPPORTED, "512 (1024) adjoint threads per fibre take fibres of <= 2048 nodes");
+      s->tune_adjoint_threads = value;
+      return 0;
     case FMS_TUNE_PROFILE_KERNELS:
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "kernel profiling is 0 or 1");
       for (hipEvent_t &e : s->prof_ev)
@@ -1497,36 +1521,53 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     s->spikes = 0;
     const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
-    const size_t lds = (size_t)(fms::pcr_lds_doubles<T>() + 2 * s->nmax) * sizeof(double);
+    const size_t lds = (size_t)(fms::pcr_lds_doubles<1024>() + 2 * s->nmax) * sizeof(double);
     hipStream_t main_stream = s->stream;
     for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
       const int buf = idx & 1;
       double *tape = s->d_tape + buf * tape_len;
       // recompute the segment from its checkpoint on stream2, recording the tape;
-      // the buffer was last read by the reverse of segment seg+2
-      s->stream = s->stream2;
+      // the buffer was last read by the reverse of segment seg+2 (serialised
+      // schedule: on the solver's own stream, before the reverse sweep)
+      if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->stream = s->stream2;
       if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess) rc = FMS_ERR_HIP;
       if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), false);
       fms::DevTrace tr{};
       tr.tape = tape;
+      tr.raise_priority = s->tune_adjoint == FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY;
       const long long save = s->step_counter;
       if (!rc) rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
       s->step_counter = save;
-      if (!rc && hipEventRecord(s->adj_ev[2 + buf], s->stream2) != hipSuccess) rc = FMS_ERR_HIP;
+      if (!rc && hipEventRecord(s->adj_ev[2 + buf], s->stream) != hipSuccess) rc = FMS_ERR_HIP;
       s->stream = main_stream;
       if (rc) break;
       // reverse the segment on the solver's stream once its tape is recorded
       HIP_TRY(hipStreamWaitEvent(s->stream, s->adj_ev[2 + buf], 0));
       const double *dtr = d_dtraces + (size_t)k0 * F * S;
-      if (s->nmax <= 4 * T)
+      const int prio = s->tune_adjoint == FMS_ADJOINT_OVERLAP_PRIORITY;
+      const int at = s->tune_adjoint_threads;
+#define FMS_ADJ_LAUNCH(TT, NPT_, MMAX_)                                                                           \
+  hipLaunchKernelGGL((fms::k_adj_fused<E, TT, NPT_, MMAX_>), dim3(F), dim3(TT), lds, s->stream, s->fibers(),       \
+                     s->stim(), tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam, \
+                     s->d_gacc, s->d_pfbar, prio)
+      if (at == 1024 && s->nmax <= 1024)
+        FMS_ADJ_LAUNCH(1024, 1, 4);
+      else if (at == 1024)
+        FMS_ADJ_LAUNCH(1024, 2, 4);
+      else if (at == 512 && s->nmax <= 1024)
+        FMS_ADJ_LAUNCH(512, 2, 4);
+      else if (at == 512)
+        FMS_ADJ_LAUNCH(512, 4, 4);
+#undef FMS_ADJ_LAUNCH
+      else if (s->nmax <= 4 * T)
         hipLaunchKernelGGL((fms::k_adj_fused<E, T, 4, 8>), dim3(F), dim3(T), lds, s->stream, s->fibers(), s->stim(),
                            tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
-                           s->d_gacc, s->d_pfbar);
+                           s->d_gacc, s->d_pfbar, prio);
       else
         hipLaunchKernelGGL((fms::k_adj_fused<E, T, 8, 8>), dim3(F), dim3(T), lds, s->stream, s->fibers(), s->stim(),
                            tape, s->d_steps + 4 * (size_t)k0, kc, n0D, ie, s->d_sensor_slot, dtr, S, s->d_lam,
-                           s->d_gacc, s->d_pfbar);
+                           s->d_gacc, s->d_pfbar, prio);
       if (hipGetLastError() != hipSuccess) {
         rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
         break;

@@ -167,17 +167,22 @@ class FiberBatch:
     def set_tuning(self, steps_per_launch: int | None = None, resident_gates: int | None = None,
                    threads: int | None = None, graph_steps: int | None = None,
                    reaction_per_fibre: int | None = None, pair_halves: int | None = None,
-                   reaction_threads: int | None = None) -> None:
+                   reaction_threads: int | None = None, adjoint: int | None = None,
+                   adjoint_threads: int | None = None) -> None:
         """Schedule knobs. Fused kernel: split steps per launch; gates in registers (1),
         streamed (0) or auto (-1); threads per fibre workgroup (128 or 256). Split
         kernels: steps per captured hipGraph (0 launches eagerly); for large models
         the reaction with one workgroup per fibre (1) or one lane per node (0); the
         two reaction halves between two solves in one launch (1, default) or one
-        launch per half (0); threads per fibre of the per-fibre reaction (256 or 64)."""
+        launch per half (0); threads per fibre of the per-fibre reaction (256 or 64).
+        Adjoint backward: segment recompute beside the reverse sweep (0), the same
+        with the reverse sweep's priority raised (1), or one after the other (2);
+        threads per fibre workgroup of the reverse sweep (256, 512 or 1024)."""
         for key, value in ((L.FMS_TUNE_STEPS_PER_LAUNCH, steps_per_launch), (L.FMS_TUNE_RESIDENT_GATES, resident_gates),
                            (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps),
                            (L.FMS_TUNE_REACTION_PER_FIBRE, reaction_per_fibre),
-                           (L.FMS_TUNE_PAIR_HALVES, pair_halves), (L.FMS_TUNE_REACTION_THREADS, reaction_threads)):
+                           (L.FMS_TUNE_PAIR_HALVES, pair_halves), (L.FMS_TUNE_REACTION_THREADS, reaction_threads),
+                           (L.FMS_TUNE_ADJOINT, adjoint), (L.FMS_TUNE_ADJOINT_THREADS, adjoint_threads)):
             if value is not None:
                 self._chk(self.lib.fms_set_tuning(self.handle, key, int(value)), "fms_set_tuning")
 

@@ -34,8 +34,8 @@
  *   static void model_rhs_e(...)                          libm exp/log
  *   static const double model_y0[NS];                      initial states
  *   MODEL_NAME                                              string literal
- * and optionally FMSO_MODEL_RHS_V4, the name of a four-lane form of model_rhs_a
- * over fmso_v4 (declared below). The reaction then runs four nodes per vector
+ * and optionally FMSO_MODEL_RHS_VW, the name of a four-lane form of model_rhs_a
+ * over fmso_vw (declared below). The reaction then runs four nodes per vector
  * operation, as the reference's Vc code runs W nodes per double_v
  * (GEN/03_generator_vc.cpp:1017-1130), each lane with the scalar operations.
  * Compile with -ffp-contract=off so every + - * / rounds as written.
@@ -119,66 +119,70 @@ static inline double fmso_log_accurate(double x)
   return dk * 0.6931471803691238 + (log_m + dk * 1.9082149292705877e-10);
 }
 
-/* four lanes of doubles (GCC vector extension): element-wise IEEE operations,
-   the W = 4 double_v of the reference's Vc build on AVX2 */
-typedef double fmso_v4 __attribute__((vector_size(32)));
+/* FMSO_W lanes of doubles (GCC vector extension): element-wise IEEE
+   operations, the double_v of the reference's Vc build: W = 4 on AVX2 (the
+   default), W = 8 on AVX-512 (built with -DFMSO_W=8) */
+#ifndef FMSO_W
+#define FMSO_W 4
+#endif
+typedef double fmso_vw __attribute__((vector_size(8 * FMSO_W)));
 
-static inline fmso_v4 exponential_approx_v4(fmso_v4 x)
+static inline fmso_vw exponential_approx_vw(fmso_vw x)
 {
   x = 1.0 + x / 1024.;
   for (int i = 0; i < 10; i++)
     x *= x;
   return x;
 }
-static inline fmso_v4 pow3_v4(fmso_v4 x) { return x * (x * x); }
-static inline fmso_v4 pow4_v4(fmso_v4 x) { fmso_v4 s = x * x; return s * s; }
+static inline fmso_vw pow3_vw(fmso_vw x) { return x * (x * x); }
+static inline fmso_vw pow4_vw(fmso_vw x) { fmso_vw s = x * x; return s * s; }
 
 /* lane masks of a comparison and the lane select m ? a : b */
-typedef __typeof__((fmso_v4){0} > (fmso_v4){0}) fmso_v4m;
-static inline fmso_v4 fmso_vsel(fmso_v4m m, fmso_v4 a, fmso_v4 b)
+typedef __typeof__((fmso_vw){0} > (fmso_vw){0}) fmso_vwm;
+static inline fmso_vw fmso_vsel(fmso_vwm m, fmso_vw a, fmso_vw b)
 {
-  return (fmso_v4)((m & (fmso_v4m)a) | (~m & (fmso_v4m)b));
+  return (fmso_vw)((m & (fmso_vwm)a) | (~m & (fmso_vwm)b));
 }
 
 /* scalar-or-lane helpers of the four-lane model forms: broadcast a scalar, and
    c ? a : b with c a scalar condition or a lane mask */
-static inline fmso_v4 fmso_v4_id(fmso_v4 x) { return x; }
-static inline fmso_v4 fmso_v4_bc(double x) { return (fmso_v4){x, x, x, x}; }
-static inline fmso_v4 fmso_v4_sel_s(int c, fmso_v4 a, fmso_v4 b) { return c ? a : b; }
-#define FMSO_BC(x) _Generic((x), fmso_v4: fmso_v4_id, default: fmso_v4_bc)(x)
-#define FMSO_SEL(c, a, b) _Generic((c), int: fmso_v4_sel_s, default: fmso_vsel)((c), FMSO_BC(a), FMSO_BC(b))
+static inline fmso_vw fmso_vw_id(fmso_vw x) { return x; }
+static inline fmso_vw fmso_vw_bc(double x) { return (fmso_vw){0} + x; }
+static inline fmso_vw fmso_vw_sel_s(int c, fmso_vw a, fmso_vw b) { return c ? a : b; }
+#define FMSO_BC(x) _Generic((x), fmso_vw: fmso_vw_id, default: fmso_vw_bc)(x)
+#define FMSO_SEL(c, a, b) _Generic((c), int: fmso_vw_sel_s, default: fmso_vsel)((c), FMSO_BC(a), FMSO_BC(b))
 
-static inline fmso_v4 fmso_log_accurate_v4(fmso_v4 x)
+static inline fmso_vw fmso_log_accurate_vw(fmso_vw x)
 {
-  fmso_v4 r;
-  for (int l = 0; l < 4; l++) r[l] = fmso_log_accurate(x[l]);
+  fmso_vw r;
+  for (int l = 0; l < FMSO_W; l++) r[l] = fmso_log_accurate(x[l]);
   return r;
 }
 
-static inline fmso_v4 logarithmic_approx_v4(fmso_v4 x)
+static inline fmso_vw logarithmic_approx_vw(fmso_vw x)
 {
   /* logarithmic_approx per lane: the three Taylor branches selected by lane */
-  fmso_v4 t1 = x - 1;
-  fmso_v4 t1_2 = t1 * t1;
-  fmso_v4 t1_4 = t1_2 * t1_2;
-  fmso_v4 t1_6 = t1_4 * t1_2;
-  fmso_v4 result1 = t1 - 0.5 * t1_2 + 1. / 3 * t1_2 * t1 - 0.25 * t1_4 + 0.2 * t1_4 * t1 - 1. / 6 * t1_6;
-  fmso_v4 t3 = x - 3;
-  fmso_v4 t3_2 = t3 * t3;
-  fmso_v4 t3_4 = t3_2 * t3_2;
-  fmso_v4 result3 = 1.0986122886681098 + 1. / 3 * t3 - 0.05555555555555555 * t3_2 + 0.012345679012345678 * t3_2 * t3 - 0.0030864197530864196 * t3_4;
-  fmso_v4 t9 = x - 9;
-  fmso_v4 t9_2 = t9 * t9;
-  fmso_v4 t9_4 = t9_2 * t9_2;
-  fmso_v4 result9 = 2.1972245773362196 + 1. / 9 * t9 - 0.006172839506172839 * t9_2 + 0.0004572473708276177 * t9_2 * t9 - 3.8103947568968146e-05 * t9_4;
-  fmso_v4 result = fmso_vsel((x >= 2) & (x < 6), result3, result1);
+  fmso_vw t1 = x - 1;
+  fmso_vw t1_2 = t1 * t1;
+  fmso_vw t1_4 = t1_2 * t1_2;
+  fmso_vw t1_6 = t1_4 * t1_2;
+  fmso_vw result1 = t1 - 0.5 * t1_2 + 1. / 3 * t1_2 * t1 - 0.25 * t1_4 + 0.2 * t1_4 * t1 - 1. / 6 * t1_6;
+  fmso_vw t3 = x - 3;
+  fmso_vw t3_2 = t3 * t3;
+  fmso_vw t3_4 = t3_2 * t3_2;
+  fmso_vw result3 = 1.0986122886681098 + 1. / 3 * t3 - 0.05555555555555555 * t3_2 + 0.012345679012345678 * t3_2 * t3 - 0.0030864197530864196 * t3_4;
+  fmso_vw t9 = x - 9;
+  fmso_vw t9_2 = t9 * t9;
+  fmso_vw t9_4 = t9_2 * t9_2;
+  fmso_vw result9 = 2.1972245773362196 + 1. / 9 * t9 - 0.006172839506172839 * t9_2 + 0.0004572473708276177 * t9_2 * t9 - 3.8103947568968146e-05 * t9_4;
+  fmso_vw result = fmso_vsel((x >= 2) & (x < 6), result3, result1);
   result = fmso_vsel(x >= 6, result9, result);
-  result -= (1 - x / exponential_approx_v4(result));
+  result -= (1 - x / exponential_approx_vw(result));
   return result;
 }
-#ifdef FMSO_MODEL_RHS_V4
-static inline void FMSO_MODEL_RHS_V4(const fmso_v4 *y, const fmso_v4 *p, double t, fmso_v4 *r,
-                                     fmso_v4 *a);
+#ifdef FMSO_MODEL_RHS_VW
+static inline void FMSO_MODEL_RHS_VW(const fmso_vw *y, const fmso_vw *p, double t, fmso_vw *r,
+                                     fmso_vw *a);
 #endif
 
 /* integer powers, built as the generator builds them: powN = pow(N/2)(pow(N-N/2)) */
@@ -379,32 +383,36 @@ static void fmso_compute0D_fiber(const fmso_problem *p, fmso_state *s, int f, lo
     if (p->stim_mode == FMSO_STIM_CURRENT && NP > 0)
       prm_on[p->stim_param] = on ? p->stim_amplitude[f] : prm[p->stim_param];
     int i = 0;
-#ifdef FMSO_MODEL_RHS_V4
-    /* models with a four-lane right-hand side: groups of four nodes, each lane
+#ifdef FMSO_MODEL_RHS_VW
+    /* models with a W-lane right-hand side: groups of FMSO_W nodes, each lane
        the scalar Heun step's operations (approximate exponential only) */
     if (p->exp_mode == FMSO_EXP_APPROX) {
-      for (; i + 4 <= n; i += 4) {
-        fmso_v4 y[NS], r[NS], ys[NS], rs[NS], a[NA > 0 ? NA : 1], as[NA > 0 ? NA : 1], pl[NP > 0 ? NP : 1];
-        int clamp[4];
-        const double *pp[4];
-        for (int k = 0; k < NS; k++) memcpy(&y[k], s->states + (long)k * ntot + off + i, sizeof(fmso_v4));
-        for (int l = 0; l < 4; l++) {
+      for (; i + FMSO_W <= n; i += FMSO_W) {
+        fmso_vw y[NS], r[NS], ys[NS], rs[NS], a[NA > 0 ? NA : 1], as[NA > 0 ? NA : 1], pl[NP > 0 ? NP : 1];
+        int clamp[FMSO_W];
+        const double *pp[FMSO_W];
+        for (int k = 0; k < NS; k++) memcpy(&y[k], s->states + (long)k * ntot + off + i, sizeof(fmso_vw));
+        for (int l = 0; l < FMSO_W; l++) {
           const int in_set = i + l >= p->stim_begin[f] && i + l < p->stim_end[f];
           clamp[l] = p->stim_mode == FMSO_STIM_FIRING_TABLE && on && in_set;
           pp[l] = (p->stim_mode == FMSO_STIM_CURRENT && in_set) ? prm_on : prm;
         }
-        for (int k = 0; k < NP; k++) pl[k] = (fmso_v4){pp[0][k], pp[1][k], pp[2][k], pp[3][k]};
-        FMSO_MODEL_RHS_V4(y, pl, t, r, a);
+        for (int k = 0; k < NP; k++) {
+          fmso_vw v = {0};
+          for (int l = 0; l < FMSO_W; l++) v[l] = pp[l][k];
+          pl[k] = v;
+        }
+        FMSO_MODEL_RHS_VW(y, pl, t, r, a);
         for (int k = 0; k < NS; k++) ys[k] = y[k] + dt * r[k];
-        for (int l = 0; l < 4; l++)
+        for (int l = 0; l < FMSO_W; l++)
           if (clamp[l]) ys[0][l] = p->value_for_stimulated_point;
-        FMSO_MODEL_RHS_V4(ys, pl, t, rs, as);
+        FMSO_MODEL_RHS_VW(ys, pl, t, rs, as);
         for (int k = 0; k < NS; k++) y[k] += 0.5 * dt * (r[k] + rs[k]);
-        for (int l = 0; l < 4; l++)
+        for (int l = 0; l < FMSO_W; l++)
           if (clamp[l]) y[0][l] = p->value_for_stimulated_point;
-        for (int k = 0; k < NS; k++) memcpy(s->states + (long)k * ntot + off + i, &y[k], sizeof(fmso_v4));
+        for (int k = 0; k < NS; k++) memcpy(s->states + (long)k * ntot + off + i, &y[k], sizeof(fmso_vw));
         if (store)
-          for (int l = 0; l < 4; l++) {
+          for (int l = 0; l < FMSO_W; l++) {
             double al[NA > 0 ? NA : 1];
             for (int k = 0; k < NA; k++) al[k] = as[k][l];
             fmso_store_alg(p, s, off + i + l, ntot, al);

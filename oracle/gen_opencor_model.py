@@ -12,8 +12,9 @@ parametersUsedAsAlgebraic entries come first, then parametersUsedAsConstant).
 The generated source contains the model file's text, so it is written to
 oracle/_gen (not versioned) and compiled to oracle/build/libfmso_<name>.so.
 
-usage: python gen_opencor_model.py MODEL.c NAME "alg,..." "const,..." [MARCH [scalar]]
-("scalar": without the four-lane function, one node at a time)
+usage: python gen_opencor_model.py MODEL.c NAME "alg,..." "const,..." [MARCH [scalar|w8]]
+("scalar": without the W-lane function, one node at a time; "w8": eight lanes,
+AVX-512)
 """
 import os
 import re
@@ -21,6 +22,9 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# build variants: the W = 4 lane path (AVX2, default), one node at a time, and
+# the W = 8 lane path of an AVX-512 host (the reference's Vc double_v there)
+VARIANT_FLAGS = {"": [], "scalar": ["-DFMSO_SCALAR_ONLY"], "w8": ["-DFMSO_W=8", "-march=x86-64-v4"]}
 
 
 def lane_selects(line):
@@ -45,7 +49,7 @@ def lane_selects(line):
     return line
 
 
-def main(path, name, p_alg, p_const, march="x86-64-v3", scalar_only=False):
+def main(path, name, p_alg, p_const, march="x86-64-v3", variant=""):
     text = open(path).read()
     n_alg = int(re.search(r"There are a total of (\d+) entries in the algebraic", text).group(1))
     n_states = int(re.search(r"There are a total of (\d+) entries in each of the rate and state", text).group(1))
@@ -89,7 +93,7 @@ static const double model_y0[NS] = {{{", ".join(y0.get(i, "0") for i in range(n_
 static void model_rhs_a(const double *STATES, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
 static void model_rhs_e(const double *STATES, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
 #ifndef FMSO_SCALAR_ONLY
-#define FMSO_MODEL_RHS_V4 model_rhs_a_v4
+#define FMSO_MODEL_RHS_VW model_rhs_a_vw
 #endif
 #include "fms_oracle_core.h"
 
@@ -137,32 +141,32 @@ static void model_rhs_e(const double *STATES, const double *P_, double VOI, doub
 }}
 #undef pow
 
-#ifdef FMSO_MODEL_RHS_V4
+#ifdef FMSO_MODEL_RHS_VW
 /* the same lines over four nodes per operation: states, parameters, rates and
    algebraics are four-lane vectors, CONSTANTS stay scalars (broadcast, as Vc
    broadcasts them), and each function picks its scalar or lane form by type */
-static fmso_v4 ipow_v4_(fmso_v4 x, int n)
+static fmso_vw ipow_vw_(fmso_vw x, int n)
 {{
   if (n == 1) return x;
   if (n == 2) return x * x;
   int e0 = n / 2, e1 = n - n / 2;
-  if (e0 == 1) return x * (ipow_v4_(x, e1));
-  return ipow_v4_(ipow_v4_(x, e1), e0);
+  if (e0 == 1) return x * (ipow_vw_(x, e1));
+  return ipow_vw_(ipow_vw_(x, e1), e0);
 }}
-static fmso_v4 oracle_pow_v4(fmso_v4 x, double y)
+static fmso_vw oracle_pow_vw(fmso_vw x, double y)
 {{
   if (y == (double)(int)y && y != 0) {{
     int n = (int)y;
-    return n > 0 ? ipow_v4_(x, n) : 1. / (ipow_v4_(x, -n));
+    return n > 0 ? ipow_vw_(x, n) : 1. / (ipow_vw_(x, -n));
   }}
-  fmso_v4 r;
-  for (int l = 0; l < 4; l++) r[l] = pow(x[l], y);
+  fmso_vw r;
+  for (int l = 0; l < FMSO_W; l++) r[l] = pow(x[l], y);
   return r;
 }}
-#define pow(x, y) _Generic((x), fmso_v4: oracle_pow_v4, default: oracle_pow)((x), (y))
-#define exp(x) _Generic((x), fmso_v4: exponential_approx_v4, default: exponential_approx)(x)
-#define log(x) _Generic((x), fmso_v4: logarithmic_approx_v4, default: logarithmic_approx)(x)
-static void model_rhs_a_v4(const fmso_v4 *STATES, const fmso_v4 *P_, double VOI, fmso_v4 *RATES, fmso_v4 *ALGEBRAIC)
+#define pow(x, y) _Generic((x), fmso_vw: oracle_pow_vw, default: oracle_pow)((x), (y))
+#define exp(x) _Generic((x), fmso_vw: exponential_approx_vw, default: exponential_approx)(x)
+#define log(x) _Generic((x), fmso_vw: logarithmic_approx_vw, default: logarithmic_approx)(x)
+static void model_rhs_a_vw(const fmso_vw *STATES, const fmso_vw *P_, double VOI, fmso_vw *RATES, fmso_vw *ALGEBRAIC)
 {{
   (void)VOI; (void)P_;
 {chr(10).join(lane_selects(b) for b in body)}
@@ -180,13 +184,14 @@ void fmso_default_params(double *p)
 """
     os.makedirs(os.path.join(HERE, "_gen"), exist_ok=True)
     os.makedirs(os.path.join(HERE, "build"), exist_ok=True)
-    cfile = os.path.join(HERE, "_gen", f"model_{name}{'_scalar' if scalar_only else ''}.c")
+    sfx = "_scalar" if variant == "scalar" else ("_w8" if variant == "w8" else "")
+    cfile = os.path.join(HERE, "_gen", f"model_{name}{sfx}.c")
     with open(cfile, "w") as f:
         f.write(src)
-    out = os.path.join(HERE, "build", f"libfmso_{name}{'_scalar' if scalar_only else ''}.so")
+    out = os.path.join(HERE, "build", f"libfmso_{name}{sfx}.so")
     subprocess.run(["gcc", "-O3", f"-march={march}", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
                     "-shared", "-std=gnu11", "-Wno-unused-function", "-Wno-unused-variable", "-I", HERE, "-o", out,
-                    cfile, "-lm"] + (["-DFMSO_SCALAR_ONLY"] if scalar_only else []), check=True)
+                    cfile, "-lm"] + VARIANT_FLAGS[variant], check=True)
     return out
 
 
@@ -194,5 +199,5 @@ if __name__ == "__main__":
     pa = [int(x) for x in sys.argv[3].split(",") if x] if len(sys.argv) > 3 else []
     pc = [int(x) for x in sys.argv[4].split(",") if x] if len(sys.argv) > 4 else []
     march = sys.argv[5] if len(sys.argv) > 5 else "x86-64-v3"
-    scalar = len(sys.argv) > 6 and sys.argv[6] == "scalar"
+    scalar = sys.argv[6] if len(sys.argv) > 6 else ""
     print(main(sys.argv[1], sys.argv[2], pa, pc, march, scalar))

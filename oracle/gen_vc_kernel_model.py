@@ -15,7 +15,7 @@ The oracle's own Heun step does the predictor, the clamp and the second
 evaluation exactly as the kernel writes them (tk2014.cpp.disabled:298-566).
 Names are bound by macros:
 
-  * ``double_v`` -> double, or the four-lane fmso_v4 for the lane path;
+  * ``double_v`` -> double, or the four-lane fmso_vw for the lane path;
   * ``exponential`` -> exponential_approx (the kernel's own, lines 23-34);
   * ``sqr``, ``pow3``, ``pow4`` -> x*x, x*x*x, sqr(sqr(x)) (lines 5-18);
   * ``log`` (Vc::log) -> fmso_log_accurate;
@@ -26,7 +26,7 @@ Names are bound by macros:
 This is a route to the model independent of the product's tokenizing converter
 (opendihu_amd/cellml/vc_kernel.py).
 
-usage: python gen_vc_kernel_model.py KERNEL.cpp NAME [MARCH [scalar]]
+usage: python gen_vc_kernel_model.py KERNEL.cpp NAME [MARCH [scalar|w8]]
 """
 import os
 import re
@@ -34,9 +34,12 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# build variants: the W = 4 lane path (AVX2, default), one node at a time, and
+# the W = 8 lane path of an AVX-512 host (the reference's Vc double_v there)
+VARIANT_FLAGS = {"": [], "scalar": ["-DFMSO_SCALAR_ONLY"], "w8": ["-DFMSO_W=8", "-march=x86-64-v4"]}
 
 
-def main(path, name, march="x86-64-v3", scalar_only=False):
+def main(path, name, march="x86-64-v3", variant=""):
     text = open(path).read()
     k0 = text.index("compute0DInstance")
     k1 = text.index("intermediateState0", k0)
@@ -76,7 +79,7 @@ static const double model_y0[NS] = {{{y0}}};
 static void model_rhs_a(const double *states, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
 static void model_rhs_e(const double *states, const double *P_, double VOI, double *RATES, double *ALGEBRAIC);
 #ifndef FMSO_SCALAR_ONLY
-#define FMSO_MODEL_RHS_V4 model_rhs_a_v4
+#define FMSO_MODEL_RHS_VW model_rhs_a_vw
 #endif
 #include "fms_oracle_core.h"
 
@@ -114,18 +117,18 @@ static void model_rhs_e(const double *states, const double *P_, double VOI, doub
 #undef double_v
 #undef Vc_iif
 
-#ifdef FMSO_MODEL_RHS_V4
-static inline fmso_v4 sqr_v(fmso_v4 x) {{ return x * x; }}
-static inline fmso_v4 pow3_v(fmso_v4 x) {{ return x * x * x; }}
-static inline fmso_v4 pow4_v(fmso_v4 x) {{ return sqr_v(sqr_v(x)); }}
+#ifdef FMSO_MODEL_RHS_VW
+static inline fmso_vw sqr_v(fmso_vw x) {{ return x * x; }}
+static inline fmso_vw pow3_v(fmso_vw x) {{ return x * x * x; }}
+static inline fmso_vw pow4_v(fmso_vw x) {{ return sqr_v(sqr_v(x)); }}
 #define Vc_iif(c, a, b) FMSO_SEL((c), (a), (b))
-#define double_v fmso_v4
-#define sqr(x) _Generic((x), fmso_v4: sqr_v, default: sqr_s)(x)
-#define pow3(x) _Generic((x), fmso_v4: pow3_v, default: pow3_s)(x)
-#define pow4(x) _Generic((x), fmso_v4: pow4_v, default: pow4_s)(x)
-#define log(x) _Generic((x), fmso_v4: fmso_log_accurate_v4, default: fmso_log_accurate)(x)
-#define exponential(x) _Generic((x), fmso_v4: exponential_approx_v4, default: exponential_approx)(x)
-static void model_rhs_a_v4(const fmso_v4 *states, const fmso_v4 *P_, double VOI, fmso_v4 *RATES, fmso_v4 *ALGEBRAIC)
+#define double_v fmso_vw
+#define sqr(x) _Generic((x), fmso_vw: sqr_v, default: sqr_s)(x)
+#define pow3(x) _Generic((x), fmso_vw: pow3_v, default: pow3_s)(x)
+#define pow4(x) _Generic((x), fmso_vw: pow4_v, default: pow4_s)(x)
+#define log(x) _Generic((x), fmso_vw: fmso_log_accurate_vw, default: fmso_log_accurate)(x)
+#define exponential(x) _Generic((x), fmso_vw: exponential_approx_vw, default: exponential_approx)(x)
+static void model_rhs_a_vw(const fmso_vw *states, const fmso_vw *P_, double VOI, fmso_vw *RATES, fmso_vw *ALGEBRAIC)
 {{
   (void)VOI; (void)P_;
   {chr(10).join("  " + c for c in consts).strip()}
@@ -144,18 +147,18 @@ void fmso_default_params(double *p) {{ (void)p; }}
 """
     os.makedirs(os.path.join(HERE, "_gen"), exist_ok=True)
     os.makedirs(os.path.join(HERE, "build"), exist_ok=True)
-    sfx = "_scalar" if scalar_only else ""
+    sfx = "_scalar" if variant == "scalar" else ("_w8" if variant == "w8" else "")
     cfile = os.path.join(HERE, "_gen", f"model_{name}{sfx}.c")
     with open(cfile, "w") as f:
         f.write(src)
     out = os.path.join(HERE, "build", f"libfmso_{name}{sfx}.so")
     subprocess.run(["gcc", "-O3", f"-march={march}", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
                     "-shared", "-std=gnu11", "-Wno-unused-function", "-Wno-unused-variable", "-I", HERE, "-o", out,
-                    cfile, "-lm"] + (["-DFMSO_SCALAR_ONLY"] if scalar_only else []), check=True)
+                    cfile, "-lm"] + VARIANT_FLAGS[variant], check=True)
     return out
 
 
 if __name__ == "__main__":
     march = sys.argv[3] if len(sys.argv) > 3 else "x86-64-v3"
-    scalar = len(sys.argv) > 4 and sys.argv[4] == "scalar"
+    scalar = sys.argv[4] if len(sys.argv) > 4 else ""
     print(main(sys.argv[1], sys.argv[2], march, scalar))

@@ -18,7 +18,7 @@
 
 static const double model_y0[NS] = {-75, 0.05, 0.6, 0.325};
 
-/* T: double, or fmso_v4 (four nodes per operation, each lane the same IEEE
+/* T: double, or fmso_vw (four nodes per operation, each lane the same IEEE
    operations as the scalar form, as the reference's Vc double_v evaluates W nodes) */
 #define HH_RHS(NAME, T, EXPF, POW3, POW4)                                                         \
   static inline void NAME(const T *y, const T *p, double t, T *r, T *a)                         \
@@ -48,15 +48,15 @@ static const double model_y0[NS] = {-75, 0.05, 0.6, 0.325};
 static inline void model_rhs_a(const double *y, const double *p, double t, double *r, double *a);
 static inline void model_rhs_e(const double *y, const double *p, double t, double *r, double *a);
 #ifndef FMSO_SCALAR_ONLY
-#define FMSO_MODEL_RHS_V4 model_rhs_a_v4
+#define FMSO_MODEL_RHS_VW model_rhs_a_vw
 #endif
 
 #include "fms_oracle_core.h"
 
 HH_RHS(model_rhs_a, double, exponential_approx, pow3_, pow4_)
 HH_RHS(model_rhs_e, double, exp, pow3_, pow4_)
-#ifdef FMSO_MODEL_RHS_V4
-HH_RHS(model_rhs_a_v4, fmso_v4, exponential_approx_v4, pow3_v4, pow4_v4)
+#ifdef FMSO_MODEL_RHS_VW
+HH_RHS(model_rhs_a_vw, fmso_vw, exponential_approx_vw, pow3_vw, pow4_vw)
 #endif
 
 /* default parameter vector of the model file: i_Stim = 0, g_Na = 120, g_K = 36, g_L = 0.3 */

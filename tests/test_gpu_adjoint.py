@@ -208,3 +208,28 @@ def test_two_outputs_in_one_loss():
     gb = prob.backward(0.5 * (prob.forward(theta1) - target.cpu().numpy()))
     np.testing.assert_allclose(ta.grad.cpu().numpy(), ga, rtol=1e-12, atol=0)
     np.testing.assert_allclose(tb.grad.cpu().numpy(), gb, rtol=1e-12, atol=0)
+
+
+def test_adjoint_schedules_give_the_same_gradient():
+    """the segment recompute beside the reverse sweep (with or without raised
+    priority) or before it: bit-identical gradients"""
+    from opendihu_amd import _lib as L
+    prob, theta_true, theta0 = _problem(n_steps=500, interval=64)
+    target = prob.forward(theta_true)
+    tr = prob.forward(theta0)
+    grads = []
+    for mode in (L.FMS_ADJOINT_OVERLAP, L.FMS_ADJOINT_OVERLAP_PRIORITY, L.FMS_ADJOINT_SERIAL,
+                 L.FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY):
+        prob.batch.set_tuning(adjoint=mode)
+        grads.append(prob.backward(tr - target))
+    for g in grads[1:]:
+        np.testing.assert_array_equal(g, grads[0])
+    choice = prob.tune_backward(theta0, tr - target)
+    assert choice["schedule"] in ("overlap", "overlap_priority", "serial", "overlap_recompute_priority")
+    # the reverse sweep with 256 or 512 threads per fibre: the partitioned solve
+    # takes other chunks and the prefactor gradient is summed over another
+    # number of waves, so the gradients agree to rounding
+    prob.batch.set_tuning(adjoint_threads=256)
+    g256 = prob.backward(tr - target)
+    prob.batch.set_tuning(adjoint_threads=512)
+    np.testing.assert_allclose(g256, grads[0], rtol=1e-9)

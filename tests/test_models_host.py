@@ -182,16 +182,58 @@ def test_tk2014_four_lane_reaction_matches_one_node_at_a_time():
     four-lane TK2014 oracle: bit-identical to the one-node-at-a-time build"""
     from cases import c2
     from oracle_binding import OracleBatch
-    for m in ("tk2014", "tk2014_scalar"):
+    from test_oracle import avx512
+    models = ("tk2014", "tk2014_scalar") + (("tk2014_w8",) if avx512() else ())
+    for m in models:
         _oracle_or_skip(m)
     mesh, stim = c2(n_fibers=2, n_nodes=103, length_cm=1.02, rate_hz=4000.0)
     out = []
-    for model in ("tk2014", "tk2014_scalar"):
+    for model in models:
         o = OracleBatch(**mesh, scheme="IE", model=model, threads=2)
         o.set_firing_table_stimulation(**stim)
         o.enable_spike_detection(-40.0)
         o.advance(0.0, 2.5e-5, 6000)
         out.append((o.states.copy(), o.spikes()[0].copy()))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for other in out[:1] + out[2:]:
+        np.testing.assert_array_equal(other[0], out[1][0])
+        np.testing.assert_array_equal(other[1], out[1][1])
     assert (out[1][1] >= 0).sum() >= 3, "the clamped nodes must fire"
+
+
+@pytest.mark.parametrize("model", ["hh", SHORTEN, "tk2014"])
+def test_model_code_under_address_and_undefined_behaviour_sanitizers(model, tmp_path):
+    """the device model code (right-hand side, Heun step with FP64 and FP32
+    gates, the clamp and the algebraics, row assembly, serial solve) built for
+    the host with -fsanitize=address,undefined and run over random states:
+    no out-of-bounds access and no undefined behaviour (tests/sanitize_heun.cpp)"""
+    import glob
+    import shutil
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    if model == "hh":
+        header, mtype = "model_hh.h", "fms::ModelHH"
+        inc = []
+    else:
+        gen = os.path.join(REPO, "opendihu_amd", "_gen")
+        found = sorted(glob.glob(os.path.join(gen, f"model_{model}_*.h")))
+        if not found:
+            pytest.skip(f"generated {model} header not present (built where the reference checkout exists)")
+        header = os.path.basename(found[0])
+        mtype = "fms::Model_" + header[len("model_"):-2]
+        inc = [f"-I{gen}"]
+    exe = tmp_path / "sanitize_heun"
+    cmd = [hipcc, "--offload-arch=gfx950", "-O1", "-g", "-std=c++17", "-ffp-contract=off",
+           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer",
+           f"-I{CSRC}", f"-I{os.path.join(REPO, 'include')}", *inc,
+           f'-DFMS_MODEL_HEADER="{header}"', f"-DFMS_MODEL_TYPE={mtype}",
+           "-o", str(exe), "-x", "hip", os.path.join(REPO, "tests", "sanitize_heun.cpp")]
+    build = subprocess.run(cmd, capture_output=True, text=True)
+    assert build.returncode == 0, build.stderr[-3000:]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="halt_on_error=1")
+    run = subprocess.run([str(exe), "1500" if model != "hh" else "20000"], capture_output=True, text=True, env=env,
+                         timeout=600)
+    assert run.returncode == 0, (run.stdout[-2000:], run.stderr[-4000:])
+    assert "checksum" in run.stdout
+    print(model, run.stdout.strip())

@@ -284,17 +284,27 @@ def test_c1_action_potential_propagates():
     assert 0.05 < v < 1.0, v
 
 
+def avx512() -> bool:
+    """the host runs AVX-512 code (the eight-lane oracle builds)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            return any(line.startswith("flags") and " avx512f" in line for line in f)
+    except OSError:
+        return False
+
+
 @pytest.mark.parametrize("stim", ["current", "firing_table"])
 def test_four_lane_reaction_matches_one_node_at_a_time(stim):
     """the HH oracle evaluates four nodes per vector operation (approximate
-    exponential); the same file built one node at a time gives bit-identical
+    exponential), eight in its AVX-512 build (where the host has AVX-512); the
+    same file built one node at a time gives bit-identical
     states and spike steps, with stimulated ranges that straddle lane groups,
     fibre lengths that leave a scalar tail, and clamped lanes"""
     n = np.array([23, 37, 41, 8], np.int32)
     mesh = dict(n_nodes=n, element_lengths=np.full(int((n - 1).sum()), 0.01),
                 prefactor=np.array([0.0077, 0.0061, 0.0090, 0.0077]))
     out = []
-    for model in ("hh", "hh_scalar"):
+    for model in ("hh", "hh_scalar") + (("hh_w8",) if avx512() else ()):
         o = OracleBatch(**mesh, scheme="CN", model=model)
         if stim == "current":
             o.set_current_stimulation(stim_begin=np.array([5, 0, 38, 3]), stim_end=np.array([9, 3, 41, 8]),
@@ -307,8 +317,9 @@ def test_four_lane_reaction_matches_one_node_at_a_time(stim):
         o.enable_spike_detection(0.0)
         o.advance(0.0, 1e-3, 1500)
         out.append((o.states.copy(), o.spikes()[0].copy()))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for other in out[:1] + out[2:]:
+        np.testing.assert_array_equal(other[0], out[1][0])
+        np.testing.assert_array_equal(other[1], out[1][1])
     assert (out[1][1] >= 0).sum() > 20, "the fibres must fire"
 
 
@@ -318,17 +329,19 @@ def test_shorten_four_lane_reaction_matches_one_node_at_a_time():
     steps four nodes at a time and one node at a time, through the upstroke of
     propagating action potentials"""
     from cases import c2
-    paths = [os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}{s}.so") for s in ("", "_scalar")]
+    variants = ("", "_scalar") + (("_w8",) if avx512() else ())
+    paths = [os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}{s}.so") for s in variants]
     if not all(os.path.exists(p) for p in paths):
         pytest.skip("generated oracle models not built (needs the reference checkout)")
     mesh, stim = c2(n_fibers=2, n_nodes=203, length_cm=2.02, rate_hz=4000.0)
     out = []
-    for model in (SHORTEN, SHORTEN + "_scalar"):
+    for model in [SHORTEN + v for v in variants]:
         o = OracleBatch(**mesh, scheme="IE", model=model, threads=2)
         o.set_firing_table_stimulation(**stim)
         o.enable_spike_detection(-40.0)
         o.advance(0.0, 2.5e-5, 12000)
         out.append((o.states.copy(), o.spikes()[0].copy()))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for other in out[:1] + out[2:]:
+        np.testing.assert_array_equal(other[0], out[1][0])
+        np.testing.assert_array_equal(other[1], out[1][1])
     assert (out[1][1] >= 0).sum() > 10, "the fibres must fire"
