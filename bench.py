@@ -428,8 +428,10 @@ def main(argv=None):
     wall, ev_ms, launches = time_gpu(g, dt, args.steps, args.warmup)
     n_total = g.n_total
     value = n_total * args.steps / wall
-    # dominant kernel: launch durations from HIP events around each launch on the
-    # solver's stream, in a separate pass after the timed one
+    # dominant kernel: launch durations from a start/stop event pair bound to each
+    # launch on the solver's stream (the kernel's own begin and end), over 64 more
+    # steps of the same schedule right after the timed ones (eager: graph replays
+    # are not timed per launch)
     g.profile_kernels(True)
     g.set_tuning(graph_steps=0)   # graph replays are not timed per launch; the kernels are the same
     g.kernel_times(reset=True)

@@ -205,7 +205,8 @@ int fms_set_tuning(FmsSolver *solver, int key, int value);
    step of its first stimulation on). */
 int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [n_fibers] */);
 
-#define FMS_TUNE_PROFILE_KERNELS 7    /* 1: time every launch with HIP events (serialises the host with the GPU) */
+#define FMS_TUNE_PROFILE_KERNELS 7    /* 1: time every launch with a start/stop event pair bound to the kernel
+                                         (hipExtLaunchKernel); the host does not wait for the launches */
 #define FMS_TUNE_PAIR_HALVES 8        /* split kernels: 1 (default) runs the second reaction half of step q and the
                                          first of step q+1 (no solve between them) as one launch that reads and
                                          writes the states once; 0: one launch per half. Same arithmetic. */
@@ -229,8 +230,9 @@ int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [
                                          workgroup, 256 (default) or 64 (one wave: four fibres share a CU and their
                                          loads do not coincide). Same arithmetic. */
 
-/* accumulated launch durations in ms and launch counts per kernel class while
-   FMS_TUNE_PROFILE_KERNELS is on (graph replays are not timed); reset != 0
+/* accumulated launch durations in ms (each the kernel's own begin to end) and
+   launch counts per kernel class while FMS_TUNE_PROFILE_KERNELS is on (graph
+   replays are not timed); waits for the launches recorded so far; reset != 0
    clears them */
 #define FMS_KERNEL_REACTION 0
 #define FMS_KERNEL_DIFFUSION 1

@@ -5,6 +5,7 @@
 // library per CellML file (fast_monodomain_solver_gpu.tpp:12-206) - here ahead
 // of time with hipcc for gfx950 instead of g++ -fopenmp at run time.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <type_traits>
@@ -160,10 +161,17 @@ struct FmsSolver {
   int tune_adjoint = FMS_ADJOINT_SERIAL;    // adjoint backward: recompute beside the reverse sweep or before it
   int tune_adjoint_threads = 512;           // threads per fibre workgroup of the reverse sweep (256, 512, 1024)
 
-  // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): HIP events
-  // around every launch on the solver's stream, waited for one by one
+  // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): a start and
+  // a stop event bound to each launch (hipExtLaunchKernel: the events take the
+  // kernel's own begin and end, not the stream's state around it), from a pool
+  // of PROF_POOL pairs that is read back when it is full and by fms_kernel_times
+  static constexpr int PROF_POOL = 256;
   int prof = 0;
-  hipEvent_t prof_ev[2] = {};
+  hipEvent_t prof_ev[2 * PROF_POOL] = {};
+  int prof_kind[PROF_POOL] = {};
+  int prof_pending = 0;   // pairs recorded and not read back
+  int prof_slot = -1;     // pair of the launch being enqueued, or -1
+  int prof_launched = 0;  // FMS_LAUNCH bound that pair to a kernel
   double prof_ms[FMS_KERNEL_CLASSES] = {};
   long long prof_n[FMS_KERNEL_CLASSES] = {};
 
@@ -297,23 +305,51 @@ int ensure_flags(FmsSolver *s, int n0D) {
   return 0;
 }
 
-// duration of one launch of a kernel class when per-kernel timing is on
+// adds the durations of the recorded launches to their classes
+void prof_drain(FmsSolver *s) {
+  for (int i = 0; i < s->prof_pending; i++) {
+    float ms = 0;
+    if (hipEventSynchronize(s->prof_ev[2 * i + 1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, s->prof_ev[2 * i], s->prof_ev[2 * i + 1]) == hipSuccess) {
+      s->prof_ms[s->prof_kind[i]] += ms;
+      s->prof_n[s->prof_kind[i]]++;
+    }
+  }
+  s->prof_pending = 0;
+}
+
+// duration of the one launch in its scope (enqueued with FMS_LAUNCH) when
+// per-kernel timing is on; the host does not wait for it
 struct KernelTimer {
   FmsSolver *s;
   int kind;
   KernelTimer(FmsSolver *s_, int kind_) : s(s_), kind(kind_) {
-    if (s->prof) (void)hipEventRecord(s->prof_ev[0], s->stream);
+    if (!s->prof) return;
+    if (s->prof_pending == FmsSolver::PROF_POOL) prof_drain(s);
+    s->prof_slot = s->prof_pending;
+    s->prof_launched = 0;
   }
   ~KernelTimer() {
-    if (!s->prof) return;
-    float ms = 0;
-    if (hipEventRecord(s->prof_ev[1], s->stream) == hipSuccess && hipEventSynchronize(s->prof_ev[1]) == hipSuccess &&
-        hipEventElapsedTime(&ms, s->prof_ev[0], s->prof_ev[1]) == hipSuccess) {
-      s->prof_ms[kind] += ms;
-      s->prof_n[kind]++;
+    if (s->prof_slot < 0) return;
+    if (s->prof_launched) {
+      s->prof_kind[s->prof_slot] = kind;
+      s->prof_pending++;
     }
+    s->prof_slot = -1;
   }
 };
+
+// a kernel launch on the solver's stream, with the timing pair of its
+// KernelTimer bound to it when per-kernel timing is on
+#define FMS_LAUNCH(s, kernel, grid, block, lds, ...)                                                         \
+  do {                                                                                                       \
+    if ((s)->prof_slot >= 0) {                                                                               \
+      hipExtLaunchKernelGGL(kernel, grid, block, lds, (s)->stream, (s)->prof_ev[2 * (s)->prof_slot],         \
+                            (s)->prof_ev[2 * (s)->prof_slot + 1], 0, __VA_ARGS__);                           \
+      (s)->prof_launched = 1;                                                                                \
+    } else                                                                                                   \
+      hipLaunchKernelGGL(kernel, grid, block, lds, (s)->stream, __VA_ARGS__);                                \
+  } while (0)
 
 // device step table: the K records of this call plus a copy of the last one
 // (the fused kernel reads one record ahead); the cursor starts at row 0 with
@@ -338,13 +374,13 @@ int upload_steps(FmsSolver *s, const double *steps, int K) {
 template <class G, class E, bool PAIR>
 void launch_reaction_fibre(FmsSolver *s, int q, int half, int n0D, const fms::DevSpikes &spk) {
   if (s->tune_reaction_threads == 64)
-    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 64, PAIR>), dim3(s->F), dim3(64), 0, s->stream,
-                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                       s->vstim, s->d_order, spk);
+    FMS_LAUNCH(s, (fms::k_reaction_fibre<Model, G, E, 64, PAIR>), dim3(s->F), dim3(64), 0,
+               s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+               s->vstim, s->d_order, spk);
   else
-    hipLaunchKernelGGL((fms::k_reaction_fibre<Model, G, E, 256, PAIR>), dim3(s->F), dim3(256), 0, s->stream,
-                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                       s->vstim, s->d_order, spk);
+    FMS_LAUNCH(s, (fms::k_reaction_fibre<Model, G, E, 256, PAIR>), dim3(s->F), dim3(256), 0,
+               s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+               s->vstim, s->d_order, spk);
 }
 
 template <class G, class E>
@@ -356,13 +392,13 @@ int launch_reaction(FmsSolver *s, int q, int half, int n0D, bool store_alg) {
   if (Model::NS > 8 && !store_alg && s->tune_reaction_fibre && !s->vc_width) {
     launch_reaction_fibre<G, E, false>(s, q, half, n0D, off);
   } else if (store_alg)
-    hipLaunchKernelGGL((fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
-                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                       s->vstim, s->d_alg, s->d_alg_idx, s->n_alg_idx, off);
+    FMS_LAUNCH(s, (fms::k_reaction<Model, G, E, true>), dim3((unsigned)blocks), dim3(threads), 0,
+               s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+               s->vstim, s->d_alg, s->d_alg_idx, s->n_alg_idx, off);
   else
-    hipLaunchKernelGGL((fms::k_reaction<Model, G, E, false>), dim3((unsigned)blocks), dim3(threads), 0, s->stream,
-                       s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
-                       s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx, off);
+    FMS_LAUNCH(s, (fms::k_reaction<Model, G, E, false>), dim3((unsigned)blocks), dim3(threads), 0,
+               s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), half, n0D,
+               s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx, off);
   HIP_TRY(hipGetLastError());
   s->n_launch_reaction++;
   return 0;
@@ -379,9 +415,9 @@ int launch_reaction_pair(FmsSolver *s, int q, int n0D) {
   } else {
     const int threads = 256;
     const long long blocks = (s->N + threads - 1) / threads;
-    hipLaunchKernelGGL((fms::k_reaction<Model, G, E, false, true>), dim3((unsigned)blocks), dim3(threads), 0,
-                       s->stream, s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), 1,
-                       n0D, s->vstim, nullptr, s->d_alg_idx, s->n_alg_idx, spk);
+    FMS_LAUNCH(s, (fms::k_reaction<Model, G, E, false, true>), dim3((unsigned)blocks), dim3(threads), 0,
+               s->fibers(), s->stim(), s->d_vm, (G *)s->d_gates, s->d_flags, step_ref(s, q), 1, n0D, s->vstim,
+               nullptr, s->d_alg_idx, s->n_alg_idx, spk);
   }
   HIP_TRY(hipGetLastError());
   s->n_launch_reaction++;
@@ -391,8 +427,8 @@ int launch_reaction_pair(FmsSolver *s, int q, int n0D) {
 int launch_flags(FmsSolver *s, int q, int half, int n0D) {
   KernelTimer timer(s, FMS_KERNEL_OTHER);
   const int threads = 64;
-  hipLaunchKernelGGL(fms::k_stim_flags, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
-                     s->stim(), s->F, s->d_flags, step_ref(s, q), half, n0D);
+  FMS_LAUNCH(s, fms::k_stim_flags, dim3((s->F + threads - 1) / threads), dim3(threads), 0,
+             s->stim(), s->F, s->d_flags, step_ref(s, q), half, n0D);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -400,8 +436,8 @@ int launch_flags(FmsSolver *s, int q, int half, int n0D) {
 int launch_flags_pair(FmsSolver *s, int q, int n0D) {
   KernelTimer timer(s, FMS_KERNEL_OTHER);
   const int threads = 64;
-  hipLaunchKernelGGL(fms::k_stim_flags_pair, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
-                     s->stim(), s->F, s->d_flags, step_ref(s, q), n0D);
+  FMS_LAUNCH(s, fms::k_stim_flags_pair, dim3((s->F + threads - 1) / threads), dim3(threads), 0,
+             s->stim(), s->F, s->d_flags, step_ref(s, q), n0D);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -411,21 +447,21 @@ int launch_thomas(FmsSolver *s, int q) {
   const int ie = s->scheme == FMS_IMPLICIT_EULER;
   if (s->thomas == FMS_THOMAS_SERIAL) {
     const int threads = 64;
-    hipLaunchKernelGGL(fms::k_thomas_serial, dim3((s->F + threads - 1) / threads), dim3(threads), 0, s->stream,
-                       s->fibers(), s->d_vm, s->d_cA, s->d_dA, step_ref(s, q), ie);
+    FMS_LAUNCH(s, fms::k_thomas_serial, dim3((s->F + threads - 1) / threads), dim3(threads), 0,
+               s->fibers(), s->d_vm, s->d_cA, s->d_dA, step_ref(s, q), ie);
   } else {
     // rows per chunk: at most ceil(nmax / 256) (>= 3 for fibres under 512 nodes);
     // the smallest register array that holds them
     const size_t lds = (size_t)(fms::pcr_lds_doubles<PART_T>() + s->nmax) * sizeof(double);
     if (s->nmax <= 1024)
-      hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 4>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
-                         s->d_vm, step_ref(s, q), ie);
+      FMS_LAUNCH(s, (fms::k_thomas_part<PART_T, 4>), dim3(s->F), dim3(PART_T), lds, s->fibers(),
+                 s->d_vm, step_ref(s, q), ie);
     else if (s->nmax <= 2048)
-      hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 8>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
-                         s->d_vm, step_ref(s, q), ie);
+      FMS_LAUNCH(s, (fms::k_thomas_part<PART_T, 8>), dim3(s->F), dim3(PART_T), lds, s->fibers(),
+                 s->d_vm, step_ref(s, q), ie);
     else
-      hipLaunchKernelGGL((fms::k_thomas_part<PART_T, 16>), dim3(s->F), dim3(PART_T), lds, s->stream, s->fibers(),
-                         s->d_vm, step_ref(s, q), ie);
+      FMS_LAUNCH(s, (fms::k_thomas_part<PART_T, 16>), dim3(s->F), dim3(PART_T), lds, s->fibers(),
+                 s->d_vm, step_ref(s, q), ie);
   }
   HIP_TRY(hipGetLastError());
   s->n_launch_thomas++;
@@ -436,8 +472,8 @@ int launch_spikes(FmsSolver *s, int q) {
   KernelTimer timer(s, FMS_KERNEL_OTHER);
   const int threads = 256;
   const long long blocks = (s->N + threads - 1) / threads;
-  hipLaunchKernelGGL(fms::k_spikes, dim3((unsigned)blocks), dim3(threads), 0, s->stream, s->N, s->d_vm,
-                     s->d_vprev, s->d_sfirst, s->d_scount, s->spike_th, step_ref(s, q));
+  FMS_LAUNCH(s, fms::k_spikes, dim3((unsigned)blocks), dim3(threads), 0, s->N, s->d_vm,
+             s->d_vprev, s->d_sfirst, s->d_scount, s->spike_th, step_ref(s, q));
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -585,10 +621,9 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const 
   const fms::DevSpikes spk = dev_spikes(s);
   const int red = s->thomas == FMS_THOMAS_SERIAL ? 16 * T : fms::pcr_lds_doubles<T>();
   const size_t lds = (size_t)(red + s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
-  hipLaunchKernelGGL((fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->stream, s->fibers(),
-                     s->stim(), s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim,
-                     s->scheme == FMS_IMPLICIT_EULER ? 1 : 0, s->thomas == FMS_THOMAS_SERIAL ? 1 : 0, s->d_order,
-                     spk, tr);
+  FMS_LAUNCH(s, (fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->fibers(), s->stim(),
+             s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim, s->scheme == FMS_IMPLICIT_EULER ? 1 : 0,
+             s->thomas == FMS_THOMAS_SERIAL ? 1 : 0, s->d_order, spk, tr);
   HIP_TRY(hipGetLastError());
   s->step_counter += K;
   s->n_launch_fused++;
@@ -1352,6 +1387,7 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "kernel profiling is 0 or 1");
       for (hipEvent_t &e : s->prof_ev)
         if (value && !e) HIP_TRY(hipEventCreate(&e));
+      if (!value) prof_drain(s);
       s->prof = value;
       return 0;
     default:
@@ -1418,6 +1454,7 @@ int fms_event_elapsed_ms(FmsSolver *s, int from, int to, double *ms) {
 int fms_kernel_times(FmsSolver *s, double *ms, long long *launches, int reset) {
   FMS_API_BEGIN
   if (!s) return fail(FMS_ERR_ARGUMENT, "null solver");
+  prof_drain(s);
   for (int k = 0; k < FMS_KERNEL_CLASSES; k++) {
     if (ms) ms[k] = s->prof_ms[k];
     if (launches) launches[k] = s->prof_n[k];

@@ -224,3 +224,46 @@ def test_autotune_rejects_schedules_with_other_bits():
     assert choice["rejected"] == len(choice["candidates_ms"]) >= 3, choice
     assert all(m == -1.0 for m in choice["candidates_ms"]) and choice["ms_per_step"] == -1.0
     assert choice["kernels"] == "fused" and choice["threads"] == 128
+
+
+def test_kernel_timing_changes_no_bit():
+    """per-launch timing (a start/stop event pair bound to each launch, read
+    back when the pool of 256 pairs is full and by kernel_times) launches the
+    same kernels: states bit-identical with it on and off; every launch is
+    counted once, with a positive duration, and the durations of a call add up
+    to less than its wall-clock time"""
+    import time
+    mesh, stim = uneven(16, lo=300, hi=900, seed=4)
+    states = {}
+    for prof in (0, 1):
+        for kernels in ("split", "fused"):
+            g = _batch(mesh, scheme="CN", kernels=kernels, thomas="partitioned")
+            g.set_current_stimulation(**stim)
+            g.profile_kernels(bool(prof))
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.advance(0.0, 1e-3, 300)   # > 256 timed launches on the split kernels
+            g.synchronize()
+            wall_ms = (time.perf_counter() - t0) * 1e3
+            times = g.kernel_times(reset=True)
+            g.advance(0.3, 1e-3, 7)
+            times2 = g.kernel_times()
+            states[(prof, kernels)] = g.get_states()
+            g.close()
+            print(prof, kernels, times, times2)
+            if not prof:
+                assert all(v == (0.0, 0) for v in times.values())
+                continue
+            assert sum(ms for ms, _ in times.values()) < wall_ms
+            if kernels == "split":
+                # paired halves: the call's first and last half alone, K - 1 pairs,
+                # K solves, a stimulation-flag launch per reaction launch
+                assert times["reaction"][1] == 2 and times["reaction_pair"][1] == 299
+                assert times["diffusion"][1] == 300 and times["other"][1] == 301
+                assert times2["reaction_pair"][1] == 6 and times2["diffusion"][1] == 7
+            else:
+                assert times["fused"][1] >= 1 and times2["fused"][1] >= 1
+            for k, (ms, n) in list(times.items()) + list(times2.items()):
+                assert (ms > 0) == (n > 0), (k, ms, n)
+    for kernels in ("split", "fused"):
+        np.testing.assert_array_equal(states[(0, kernels)], states[(1, kernels)], err_msg=kernels)
