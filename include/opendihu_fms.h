@@ -215,7 +215,9 @@ int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [
    of the next segment runs on a second stream beside the reverse of this one;
    _OVERLAP_PRIORITY: the same, the reverse sweep's waves with raised issue
    priority; _SERIAL (default): recompute and reverse one after the other on the
-   solver's stream. Same arithmetic, same gradients. */
+   solver's stream. A recompute beside the reverse sweep streams its gates from
+   HBM (fewer registers, so that it shares the SIMDs with the sweep); a serial
+   one uses the solver's fused schedule. Same arithmetic, same gradients. */
 #define FMS_TUNE_ADJOINT 10
 #define FMS_ADJOINT_OVERLAP 0
 #define FMS_ADJOINT_OVERLAP_PRIORITY 1

@@ -97,26 +97,30 @@ class AdjointProblem:
                 "fms_adjoint_backward_device")
         return torch.cat([gp, (gpf / self.am_cm)[:, None]], dim=1)
 
-    def tune_backward(self, theta, dtraces) -> dict:
+    def tune_backward(self, theta, dtraces, threads=(256, 512)) -> dict:
         """time fms_adjoint_backward under each adjoint schedule (segment
         recompute before the reverse sweep, or beside it with equal priority,
-        the sweep's or the recompute's priority raised) for one forward at theta
-        and keep the fastest; the gradients are the same under every schedule"""
+        the sweep's or the recompute's priority raised) and each workgroup size
+        of the reverse sweep, for one forward at theta, and keep the fastest.
+        The schedules give the same gradient; the workgroup size changes the
+        order of the per-fibre prefactor-gradient sum (last bits)."""
         import time
         from . import _lib
         times = {}
         self.forward(theta)
-        for mode in (_lib.FMS_ADJOINT_SERIAL, _lib.FMS_ADJOINT_OVERLAP, _lib.FMS_ADJOINT_OVERLAP_PRIORITY,
-                     _lib.FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY):
-            self.batch.set_tuning(adjoint=mode)
-            self.backward(dtraces)   # untimed: buffers, code objects
-            t0 = time.perf_counter()
-            self.backward(dtraces)
-            times[mode] = time.perf_counter() - t0
-        best = min(times, key=times.get)
-        self.batch.set_tuning(adjoint=best)
         names = {0: "overlap", 1: "overlap_priority", 2: "serial", 3: "overlap_recompute_priority"}
-        return dict(schedule=names[best], seconds={names[k]: v for k, v in times.items()})
+        for tt in threads:
+            for mode in (_lib.FMS_ADJOINT_SERIAL, _lib.FMS_ADJOINT_OVERLAP, _lib.FMS_ADJOINT_OVERLAP_PRIORITY,
+                         _lib.FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY):
+                self.batch.set_tuning(adjoint=mode, adjoint_threads=tt)
+                self.backward(dtraces)   # untimed: buffers, code objects
+                t0 = time.perf_counter()
+                self.backward(dtraces)
+                times[(tt, mode)] = time.perf_counter() - t0
+        best = min(times, key=times.get)
+        self.batch.set_tuning(adjoint_threads=best[0], adjoint=best[1])
+        return dict(schedule=names[best[1]], threads=best[0],
+                    seconds={f"{names[m]}/{tt}": v for (tt, m), v in times.items()})
 
     def backward(self, dtraces) -> np.ndarray:
         """dL/dtheta for the last forward; theta columns (g_Na, g_K, g_L, sigma)"""

@@ -200,37 +200,47 @@ __device__ __forceinline__ void hh_heun_step_reverse(const double *y, const doub
 }
 
 // current-pulse stimulus parameter of one Heun step, as the forward sets it
-// (the pulse is a constant of each Heun step; fms_fused.h fused_node_half)
+// (the pulse is a constant of each Heun step; fms_fused.h fused_node_half).
+// Selects per element, not a store at a run-time index, so pj stays in registers.
 template <class E>
 FMS_HD void stim_params(const DevStim &st, int f, bool inset, const double *p, double t, double *pj) {
-  for (int k = 0; k < 4; k++) pj[k] = p[k];
-  if (st.mode == 2) pj[st.stim_param] = (current_on(st, f, t) && inset) ? st.amplitude[f] : p[st.stim_param];
+  const bool pulse = st.mode == 2 && current_on(st, f, t) && inset;
+#pragma unroll
+  for (int k = 0; k < 4; k++) pj[k] = pulse && k == st.stim_param ? st.amplitude[f] : p[k];
 }
 
-// forward reaction half of one node (n0D Heun steps from t0), the forward's arithmetic
-template <class E>
-__device__ __forceinline__ void forward_half(const DevStim &st, int f, bool inset, double *y, const double *p,
-                                             double t0, double dt, int n0D) {
-  for (int j = 0; j < n0D; j++) {
-    double pj[4];
-    stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
-    heun_node<ModelHH, double, E, false>(y[0], y + 1, pj, t0 + j * dt, dt, false, 0.0);
-  }
-}
-
-// reverse of one reaction half of one node starting from state y0 at t0:
-// l[4] in: adjoint of the half's output, out: adjoint of y0; g[3] += dL/d(gNa, gK, gL).
-// Heun step j is reversed from its recomputed input with the stimulus
-// parameter of that step (a per-lane select), so all lanes take the same path.
-template <class E>
-__device__ __forceinline__ void reverse_half(const DevStim &st, int f, bool inset, const double *y0, const double *p,
-                                             double t0, double dt, int n0D, double *l, double *g) {
+// reverse of one reaction half of a thread's NPT nodes, each from its state
+// y0[q] at t0: l[q] in: adjoint of the half's output, out: adjoint of y0[q];
+// g[q] += dL/d(gNa, gK, gL). Heun step j is reversed from its recomputed input
+// with the stimulus parameter of that step (a per-lane select), so all lanes
+// take the same path. The node loop is the innermost one: the NPT nodes'
+// independent linearisations then share one basic block, which the scheduler
+// interleaves (a node loop around the run-time loop over the Heun steps gives
+// one block per node and a single dependency chain at a time).
+template <class E, int NPT>
+__device__ __forceinline__ void reverse_half(const DevStim &st, int f, const bool *inset, const double (*y0)[4],
+                                             const double *p, double t0, double dt, int n0D, double (*l)[4],
+                                             double (*g)[3]) {
   for (int j = n0D - 1; j >= 0; j--) {
-    double y[4] = {y0[0], y0[1], y0[2], y0[3]};
-    forward_half<E>(st, f, inset, y, p, t0, dt, j);
-    double pj[4];
-    stim_params<E>(st, f, inset, p, t0 + j * dt, pj);
-    hh_heun_step_reverse<E>(y, pj, dt, l, g);
+    double y[NPT][4];
+#pragma unroll
+    for (int q = 0; q < NPT; q++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) y[q][k] = y0[q][k];
+    for (int jj = 0; jj < j; jj++) {
+#pragma unroll
+      for (int q = 0; q < NPT; q++) {
+        double pj[4];
+        stim_params<E>(st, f, inset[q], p, t0 + jj * dt, pj);
+        heun_node<ModelHH, double, E, false>(y[q][0], y[q] + 1, pj, t0 + jj * dt, dt, false, 0.0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+      double pj[4];
+      stim_params<E>(st, f, inset[q], p, t0 + j * dt, pj);
+      hh_heun_step_reverse<E>(y[q], pj, dt, l[q], g[q]);
+    }
   }
 }
 
@@ -267,6 +277,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   double l[NPT][4], g[NPT][3];
   int slot[NPT];
+  bool inset[NPT];
   double pfacc = 0;
   // the fibre's running prefactor gradient, held by thread 0 for the segment:
   // the same additions in the same order as adding to pf_bar[f] every step
@@ -280,6 +291,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
 #pragma unroll
     for (int k = 0; k < 3; k++) g[q][k] = gacc[k * N + off + i];
     slot[q] = sensor_slot[off + i];
+    inset[q] = i >= sb && i < se;
   }
   for (int s = kc - 1; s >= 0; s--) {
     const double *row = steps + 4 * s;
@@ -308,12 +320,13 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
     for (int q = 0; q < NPT; q++) {
       const bool valid = tid + q * T < n;
       const int i = valid ? tid + q * T : n - 1;
-      const bool inset = i >= sb && i < se;
       l[q][0] += slot[q] >= 0 ? dl[q] : 0.0;
       if (valid) lv[i] = ie ? yb[q][0] : va[q] + yb[q][0];
-      reverse_half<E>(st, f, inset, yb[q], p, t + 0.5 * dts, dt0, n0D, l[q], g[q]);
-      if (valid) lw[i] = l[q][0];
     }
+    reverse_half<E, NPT>(st, f, inset, yb, p, t + 0.5 * dts, dt0, n0D, l, g);
+#pragma unroll
+    for (int q = 0; q < NPT; q++)
+      if (tid + q * T < n) lw[tid + q * T] = l[q][0];
     __syncthreads();
     // reverse diffusion
     lds_thomas_part<T, MMAX, true>(lw, n, h, pf, dt1, ie != 0, red, tid);
@@ -348,12 +361,7 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
     // writes red only after the barrier that follows its first loop, so thread
     // 0's read of red needs no barrier of its own)
     // reverse of half 1
-#pragma unroll
-    for (int q = 0; q < NPT; q++) {
-      const int i = tid + q * T < n ? tid + q * T : n - 1;
-      const bool inset = i >= sb && i < se;
-      reverse_half<E>(st, f, inset, y0[q], p, t, dt0, n0D, l[q], g[q]);
-    }
+    reverse_half<E, NPT>(st, f, inset, y0, p, t, dt0, n0D, l, g);
   }
   if (tid == 0) pf_bar[f] = pftotal;
 #pragma unroll

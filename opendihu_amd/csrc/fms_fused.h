@@ -48,19 +48,30 @@ __device__ __forceinline__ void tape_store_streamed(const DevTrace &tr, long lon
     for (int k = 1; k < NS; k++) t[(long long)k * N] = (double)gates[(long long)(k - 1) * gstride + i];
 }
 
-template <class M, class G, class E>
-__device__ __forceinline__ void fused_node_half(double &V, G *g, double *p, double pbase, const DevStim &st,
-                                                bool inset, const unsigned char *flags, double t0, double dt,
-                                                int n0D, double vstim) {
+// One reaction half of the NPT nodes a thread owns (n0D Heun steps each, gates
+// resident in registers). The node loop is the innermost one, so the nodes'
+// independent Heun steps share one basic block that the scheduler interleaves
+// (a node loop around the run-time loop over the Heun steps would give one
+// block, and one dependency chain, per node). Each node takes its stimulus
+// parameter by a select from the fibre's base parameters.
+template <class M, class G, class E, int NPT>
+__device__ __forceinline__ void fused_nodes_half(double *V, G (*g)[M::NS - 1], const double *p, const DevStim &st,
+                                                 const bool *inset, const unsigned char *flags, double t0, double dt,
+                                                 int n0D, double vstim) {
   for (int j = 0; j < n0D; j++) {
     // one fibre per workgroup: the flags are the same on every lane
     const int fl = __builtin_amdgcn_readfirstlane((int)flags[j]);
     if (!(fl & FLAG_COMPUTE)) continue;
     const int on = fl & FLAG_STIM;
-    const bool clamp = st.mode == 1 && on && inset;
-    // p is shared by all nodes a thread owns: set the stimulus parameter for every node
-    if (st.mode == 2 && M::NP > 0) p[st.stim_param] = (on && inset) ? st.amplitude[0] : pbase;
-    heun_node<M, G, E, false>(V, g, p, t0 + j * dt, dt, clamp, vstim);
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+      const bool clamp = st.mode == 1 && on && inset[q];
+      const bool pulse = st.mode == 2 && on && inset[q];
+      double pq[M::NP > 0 ? M::NP : 1];
+#pragma unroll
+      for (int k = 0; k < M::NP; k++) pq[k] = pulse && k == st.stim_param ? st.amplitude[0] : p[k];
+      heun_node<M, G, E, false>(V[q], g[q], pq, t0 + j * dt, dt, clamp, vstim);
+    }
   }
 }
 
@@ -126,7 +137,6 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
   for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
-  const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   const int sb = fb.stim_begin[f], se = fb.stim_end[f];
   // the current-pulse amplitude is read through a per-fibre view
   DevStim stf = st;
@@ -134,16 +144,19 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   const double *fparams = fb.params + (long long)f * (M::NP > 0 ? M::NP : 1);
   const double amp = st.mode == 2 ? st.amplitude[f] : 0.0;
 
-  G greg[NPT > 0 ? NPT : 1][M::NS - 1];
-  double vprev[NPT > 0 ? NPT : 1];
-  int sfirst[NPT > 0 ? NPT : 1], scount[NPT > 0 ? NPT : 1];
-  if (NPT > 0) {
+  constexpr int NQ = NPT > 0 ? NPT : 1;
+  G greg[NQ][M::NS - 1];
+  double vprev[NQ];
+  int sfirst[NQ], scount[NQ];
+  bool inset[NQ];
+  if constexpr (NPT > 0) {
 #pragma unroll
-    for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+    for (int q = 0; q < NQ; q++) {
       // lanes past the fibre's end shadow its last node (computed, never stored)
       const int i = tid + q * T < n ? tid + q * T : n - 1;
 #pragma unroll
       for (int k = 0; k < M::NS - 1; k++) greg[q][k] = gates[(long long)k * GS + off + i];
+      inset[q] = i >= sb && i < se;
       vprev[q] = 0;
       sfirst[q] = 0;
       scount[q] = 0;
@@ -166,16 +179,21 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     // every lane runs the reaction: lanes past the fibre's end shadow its last
     // node and store nothing (a shadow may read that node's Vm before or after
     // its owner updates it; the shadow's result is discarded)
-    if (NPT > 0) {
+    if constexpr (NPT > 0) {
+      double V[NQ];
 #pragma unroll
-      for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+      for (int q = 0; q < NQ; q++) {
         const bool valid = tid + q * T < n;
         const int i = valid ? tid + q * T : n - 1;
-        double V = lv[i];
-        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, 0, off + i, V, greg[q], M::NS);
-        fused_node_half<M, G, E>(V, greg[q], p, pbase, stf, i >= sb && i < se, sflags, t0, dt0, n0D, vstim);
-        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, M::NS, off + i, V, greg[q], M::NS);
-        if (valid) lv[i] = V;
+        V[q] = lv[i];
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, 0, off + i, V[q], greg[q], M::NS);
+      }
+      fused_nodes_half<M, G, E, NQ>(V, greg, p, stf, inset, sflags, t0, dt0, n0D, vstim);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const bool valid = tid + q * T < n;
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, 0, M::NS, off + tid + q * T, V[q], greg[q], M::NS);
+        if (valid) lv[tid + q * T] = V[q];
       }
     } else {
       for (int base = 0; base < n; base += T) {
@@ -216,37 +234,40 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     }
     __syncthreads();
 
-    // one node: second half of step s, traces and spikes, first half of step s+1
-    // (valid = false: a shadow lane, which computes and stores nothing)
-    auto node = [&](int i, bool valid, double &V, G *g, double &vp, int &sf, int &sc) {
-      if (tr.tape && valid) tape_store<M::NS>(tr, N, s, 2 * M::NS, off + i, V, g, 1);
-      fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags, t_mid, dt0, n0D, vstim);
-      if (tr.sensor_slot && valid) {
-        const int slot = tr.sensor_slot[off + i];
-        if (slot >= 0) tr.trace[((long long)s * fb.n_fibers + f) * tr.n_sensors + slot] = V;
-      }
-      if (spk.enabled) {
-        const bool cross = vp < spk.threshold && V >= spk.threshold;
-        sc += cross ? 1 : 0;
-        sf = (cross && sf < 0) ? spk.step0 + s + 1 : sf;
-        vp = V;
-      }
-      if (more) {
-        if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, 0, off + i, V, g, M::NS);
-        fused_node_half<M, G, E>(V, g, p, pbase, stf, i >= sb && i < se, sflags + n0D, st_s[4], st_s[6], n0D,
-                                 vstim);
-        if (tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, M::NS, off + i, V, g, M::NS);
-      }
-    };
-
-    if (NPT > 0) {
+    // the thread's nodes: second half of step s, traces and spikes, first half
+    // of step s+1 (a shadow lane computes and stores nothing)
+    if constexpr (NPT > 0) {
+      double V[NQ];
 #pragma unroll
-      for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+      for (int q = 0; q < NQ; q++) {
         const bool valid = tid + q * T < n;
         const int i = valid ? tid + q * T : n - 1;
-        double V = lv[i];
-        node(i, valid, V, greg[q], vprev[q], sfirst[q], scount[q]);
-        if (valid) lv[i] = V;
+        V[q] = lv[i];
+        if (tr.tape && valid) tape_store<M::NS>(tr, N, s, 2 * M::NS, off + i, V[q], greg[q], 1);
+      }
+      fused_nodes_half<M, G, E, NQ>(V, greg, p, stf, inset, sflags, t_mid, dt0, n0D, vstim);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const bool valid = tid + q * T < n;
+        const int i = valid ? tid + q * T : n - 1;
+        if (tr.sensor_slot && valid) {
+          const int slot = tr.sensor_slot[off + i];
+          if (slot >= 0) tr.trace[((long long)s * fb.n_fibers + f) * tr.n_sensors + slot] = V[q];
+        }
+        if (spk.enabled) {
+          const bool cross = vprev[q] < spk.threshold && V[q] >= spk.threshold;
+          scount[q] += cross ? 1 : 0;
+          sfirst[q] = (cross && sfirst[q] < 0) ? spk.step0 + s + 1 : sfirst[q];
+          vprev[q] = V[q];
+        }
+        if (more && tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, 0, off + i, V[q], greg[q], M::NS);
+      }
+      if (more) fused_nodes_half<M, G, E, NQ>(V, greg, p, stf, inset, sflags + n0D, st_s[4], st_s[6], n0D, vstim);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const bool valid = tid + q * T < n;
+        if (more && tr.tape && valid) tape_store<M::NS>(tr, N, s + 1, M::NS, off + tid + q * T, V[q], greg[q], M::NS);
+        if (valid) lv[tid + q * T] = V[q];
       }
     } else {
       // streamed gates (large models): one reaction half per pass over the
@@ -289,9 +310,9 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   }
 
   for (int i = tid; i < n; i += T) vm[off + i] = lv[i];
-  if (NPT > 0) {
+  if constexpr (NPT > 0) {
 #pragma unroll
-    for (int q = 0; q < (NPT > 0 ? NPT : 1); q++) {
+    for (int q = 0; q < NQ; q++) {
       const int i = tid + q * T;
       if (i < n) {
 #pragma unroll

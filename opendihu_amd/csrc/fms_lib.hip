@@ -1556,6 +1556,11 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     HIP_TRY(hipStreamWaitEvent(s->stream2, s->adj_ev[2], 0));
     const int saved_spikes = s->spikes;
     s->spikes = 0;
+    // a recompute beside the reverse sweep streams its gates: the resident-gate
+    // kernel's registers (194 per lane for HH) do not fit on a SIMD beside a
+    // reverse wave, the streamed kernel's (126) do; both give the same bits
+    const int saved_resident = s->tune_resident;
+    if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->tune_resident = 0;
     const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
     const size_t lds = (size_t)(fms::pcr_lds_doubles<1024>() + 2 * s->nmax) * sizeof(double);
@@ -1614,6 +1619,7 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     s->stream = main_stream;
     if (rc) (void)hipStreamSynchronize(s->stream2);
     s->spikes = saved_spikes;
+    s->tune_resident = saved_resident;
     if (rc) return rc;
     hipLaunchKernelGGL(fms::k_adj_reduce, dim3(F), dim3(256), 0, s->stream, s->fibers(), s->d_gacc, s->d_grad);
     HIP_TRY(hipGetLastError());

@@ -226,6 +226,7 @@ def test_adjoint_schedules_give_the_same_gradient():
         np.testing.assert_array_equal(g, grads[0])
     choice = prob.tune_backward(theta0, tr - target)
     assert choice["schedule"] in ("overlap", "overlap_priority", "serial", "overlap_recompute_priority")
+    assert choice["threads"] in (256, 512) and len(choice["seconds"]) == 8
     # the reverse sweep with 256 or 512 threads per fibre: the partitioned solve
     # takes other chunks and the prefactor gradient is summed over another
     # number of waves, so the gradients agree to rounding
