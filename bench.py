@@ -57,13 +57,17 @@ SHORTEN = "shorten_ocallaghan_davidson_soboleva_2007"
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.json")
 
 
-def measured_traffic(workload: str, kclass: str, threads: int):
-    """(bytes per node and launch, source) of the dominant kernel, or (None, None)"""
+def measured_traffic(workload: str, kclass: str, threads: int, nodes: int, node_steps_per_launch: float):
+    """(HBM bytes per launch, source) of the dominant kernel from its record
+    (bytes per node and launch, or per node-step for a fused launch of several
+    split steps), or (None, None)"""
     try:
         with open(TRAFFIC_FILE) as f:
             for r in json.load(f)["records"]:
                 if (r["workload"], r["kernel_class"], r["threads"]) == (workload, kclass, threads):
-                    return r["bytes_per_node_launch"], r["source"]
+                    if "bytes_per_node_step" in r:
+                        return r["bytes_per_node_step"] * node_steps_per_launch, r["source"]
+                    return r["bytes_per_node_launch"] * nodes, r["source"]
     except (OSError, ValueError, KeyError):
         pass
     return None, None
@@ -228,7 +232,9 @@ def bench_c3(cpu=True):
     from opendihu_amd.build import HH_LIBRARY
     dt, n_steps = 1e-3, 10000
     mesh, stim = uneven(4096, seed=3)
-    trial = step_table(0.0, dt, 64)
+    # as long as one fused launch of the timed run: shorter trials understate the
+    # fused kernel (0.181 ms/step in 64-step launches, 0.166 in 512-step ones)
+    trial = step_table(0.0, dt, 512)
     res = {}
     for precision in ("fp64", "mixed"):
         g = FiberBatch(**mesh, scheme="CN", precision=precision, kernels="fused", thomas="partitioned",
@@ -385,7 +391,7 @@ def bench_tk2014(cpu=True, steps=1000, warmup=50):
         g = FiberBatch(**mesh, scheme="IE", precision=precision, kernels="fused", thomas="partitioned",
                        library=TK2014_LIBRARY)
         g.set_firing_table_stimulation(**stim)
-        sched = g.autotune(step_table(0.0, dt, 16), allow_split=True)
+        sched = g.autotune(step_table(0.0, dt, 64), allow_split=True)
         wall, _, _ = time_gpu(g, dt, steps, warmup)
         res[precision] = dict(value=g.n_total * steps / wall, ms_per_step=wall / steps * 1e3, schedule=sched)
         g.close()
@@ -437,7 +443,7 @@ def main(argv=None):
     g, mesh, stim, desc, dt, scheme, oracle_model = make_workload(args.workload, "fp64", args.kernels)
     schedule = None
     if args.kernels == "auto":
-        schedule = g.autotune(step_table(0.0, dt, 16), allow_split=True)
+        schedule = g.autotune(step_table(0.0, dt, 64), allow_split=True)
     wall, ev_ms, launches = time_gpu(g, dt, args.steps, args.warmup)
     n_total = g.n_total
     value = n_total * args.steps / wall
@@ -449,7 +455,11 @@ def main(argv=None):
     g.set_tuning(graph_steps=0)   # graph replays are not timed per launch; the kernels are the same
     g.kernel_times(reset=True)
     t_prof = (args.warmup + args.steps) * dt
-    g.advance(t_prof, dt, 64)
+    # 64 split steps, or one whole launch of a fused schedule (its per-step cost
+    # depends on the steps per launch)
+    fused = (schedule or {}).get("kernels") == "fused"
+    prof_steps = max(64, schedule["steps_per_launch"]) if fused else 64
+    g.advance(t_prof, dt, prof_steps)
     g.synchronize()
     times = g.kernel_times()
     g.profile_kernels(False)
@@ -472,20 +482,21 @@ def main(argv=None):
         b_unit = bytes_per_node_step(g.n_states, "fp64") / halves
     else:
         kernel = "k_fused"
-        units_per_launch = n_total * 64 / max(1, n_launch)
+        units_per_launch = n_total * prof_steps / max(1, n_launch)
         f_unit = L.flops_per_node_step(g.lib, g.n_states, 1, scheme)
         b_unit = bytes_per_node_step(g.n_states, "fp64")
     achieved_tflops = f_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e12
     achieved_gbs = b_unit * units_per_launch / (mean_launch_ms * 1e-3) / 1e9
     rthreads = (schedule or {}).get("reaction_threads", 256)
-    traffic_per_node, traffic_source = measured_traffic(args.workload, kclass, rthreads)
+    node_steps_per_launch = n_total * prof_steps / max(1, n_launch)
+    traffic, traffic_source = measured_traffic(args.workload, kclass, rthreads, n_total, node_steps_per_launch)
     kernel_times = {k: dict(total_ms=v[0], launches=v[1]) for k, v in times.items()}
     g.close()
 
     mixed = None
     if not args.no_mixed:
         gm, *_ = make_workload(args.workload, "mixed", args.kernels)
-        mixed_schedule = gm.autotune(step_table(0.0, dt, 16), allow_split=True) if args.kernels == "auto" else None
+        mixed_schedule = gm.autotune(step_table(0.0, dt, 64), allow_split=True) if args.kernels == "auto" else None
         wall_m, ev_m, _ = time_gpu(gm, dt, args.steps, args.warmup)
         mixed = dict(value=n_total * args.steps / wall_m, ms_per_step=wall_m / args.steps * 1e3,
                      bytes_per_node_step=bytes_per_node_step(gm.n_states, "mixed"), dtype="f32 states + f64 Vm/Thomas",
@@ -535,13 +546,13 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": traffic_per_node * n_total if traffic_per_node else None,
+            "traffic": traffic,
             "traffic_source": traffic_source,
             "ops_per_node_and_launch_unit": f_unit,
             "nodes_x_units_per_launch": units_per_launch,
             "mean_launch_ms": mean_launch_ms,
             "launches_timed": n_launch,
-            "kernel_times_64_steps": kernel_times,
+            "kernel_times": dict(kernel_times, split_steps=prof_steps),
             "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_node_and_launch_unit": b_unit},
         },

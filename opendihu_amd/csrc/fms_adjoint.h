@@ -266,11 +266,12 @@ __global__ __launch_bounds__(T) void k_adj_fused(DevFibers fb, DevStim st, const
   const long long N = fb.n_total;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
-  const double *h = fb.elem_len + off - f;
   const double pf = fb.prefactor[f];
   double *red = sh;                          // partitioned solve's block interfaces (pcr_lds_doubles)
   double *lw = sh + pcr_lds_doubles<T>();    // n: lambda_V, then w
   double *lv = lw + n;       // n: the diffusion's V (IE: V^b, CN: V^a + V^b), for the neighbours
+  double *h = lv + n;        // n - 1 element lengths, staged once (read after the first step's barrier)
+  for (int i = tid; i < n - 1; i += T) h[i] = fb.elem_len[off - f + i];
   double p[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) p[k] = fb.params[(long long)f * 4 + k];

@@ -126,13 +126,16 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
   const long long N = fb.n_total, GS = fb.gate_stride;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
-  const double *h = fb.elem_len + off - f;
   const double pf = fb.prefactor[f];
   double *red = sh;   // block interfaces of the partitioned solve (pcr_lds_doubles), or the serial solve's cA/dA (16*T doubles)
   double *lv = sh + (serial ? 16 * T : pcr_lds_doubles<T>());   // n doubles: Vm
-  unsigned char *sflags = (unsigned char *)(lv + n);   // 2*n0D flags
+  double *h = lv + n;   // n - 1 element lengths, staged once for all the launch's solves
+  unsigned char *sflags = (unsigned char *)(h + n);   // 2*n0D flags
 
-  for (int i = tid; i < n; i += T) lv[i] = vm[off + i];
+  for (int i = tid; i < n; i += T) {
+    lv[i] = vm[off + i];
+    if (i < n - 1) h[i] = fb.elem_len[off - f + i];
+  }
 
   double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll

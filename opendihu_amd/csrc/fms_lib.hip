@@ -620,7 +620,7 @@ int launch_fused_npt(FmsSolver *s, const double *d_steps, int K, int n0D, const 
   KernelTimer timer(s, FMS_KERNEL_FUSED);
   const fms::DevSpikes spk = dev_spikes(s);
   const int red = s->thomas == FMS_THOMAS_SERIAL ? 16 * T : fms::pcr_lds_doubles<T>();
-  const size_t lds = (size_t)(red + s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
+  const size_t lds = (size_t)(red + 2 * s->nmax) * sizeof(double) + 2 * (size_t)n0D + 16;
   FMS_LAUNCH(s, (fms::k_fused<Model, G, E, T, MMAX, NPT>), dim3(s->F), dim3(T), lds, s->fibers(), s->stim(),
              s->d_vm, (G *)s->d_gates, d_steps, K, n0D, s->vstim, s->scheme == FMS_IMPLICIT_EULER ? 1 : 0,
              s->thomas == FMS_THOMAS_SERIAL ? 1 : 0, s->d_order, spk, tr);
@@ -1563,7 +1563,7 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->tune_resident = 0;
     const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
-    const size_t lds = (size_t)(fms::pcr_lds_doubles<1024>() + 2 * s->nmax) * sizeof(double);
+    const size_t lds = (size_t)(fms::pcr_lds_doubles<1024>() + 3 * s->nmax) * sizeof(double);
     hipStream_t main_stream = s->stream;
     for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
@@ -1573,7 +1573,8 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
       // the buffer was last read by the reverse of segment seg+2 (serialised
       // schedule: on the solver's own stream, before the reverse sweep)
       if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->stream = s->stream2;
-      if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess) rc = FMS_ERR_HIP;
+      if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess)
+        rc = fail(FMS_ERR_HIP, "hipStreamWaitEvent failed");
       if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), false);
       fms::DevTrace tr{};
       tr.tape = tape;
@@ -1585,7 +1586,10 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
       s->stream = main_stream;
       if (rc) break;
       // reverse the segment on the solver's stream once its tape is recorded
-      HIP_TRY(hipStreamWaitEvent(s->stream, s->adj_ev[2 + buf], 0));
+      if (hipStreamWaitEvent(s->stream, s->adj_ev[2 + buf], 0) != hipSuccess) {
+        rc = fail(FMS_ERR_HIP, "hipStreamWaitEvent failed");
+        break;
+      }
       const double *dtr = d_dtraces + (size_t)k0 * F * S;
       const int prio = s->tune_adjoint == FMS_ADJOINT_OVERLAP_PRIORITY;
       const int at = s->tune_adjoint_threads;
@@ -1614,7 +1618,10 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
         rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
         break;
       }
-      HIP_TRY(hipEventRecord(s->adj_ev[buf], s->stream));
+      if (hipEventRecord(s->adj_ev[buf], s->stream) != hipSuccess) {
+        rc = fail(FMS_ERR_HIP, "hipEventRecord failed");
+        break;
+      }
     }
     s->stream = main_stream;
     if (rc) (void)hipStreamSynchronize(s->stream2);
