@@ -491,6 +491,22 @@ def main(argv=None):
     node_steps_per_launch = n_total * prof_steps / max(1, n_launch)
     traffic, traffic_source = measured_traffic(args.workload, kclass, rthreads, n_total, node_steps_per_launch)
     kernel_times = {k: dict(total_ms=v[0], launches=v[1]) for k, v in times.items()}
+    # the same batch under the reference's default onlyComputeIfHasBeenStimulated
+    # (a fibre's reaction starts with its first stimulation): fibres whose motor
+    # unit has not fired skip their reaction work; not the headline figure, which
+    # computes every node's reaction every step
+    g.profile_kernels(False)
+    g.set_only_compute_if_stimulated(True)
+    t_oc = t_prof + prof_steps * dt
+    g.advance(t_oc, dt, 16)
+    g.synchronize()
+    w0 = time.perf_counter()
+    n_oc = min(args.steps, 1000)
+    g.advance(t_oc + 16 * dt, dt, n_oc)
+    g.synchronize()
+    only_if_stimulated = dict(ms_per_step=(time.perf_counter() - w0) / n_oc * 1e3, steps=n_oc,
+                              note="reference default onlyComputeIfHasBeenStimulated; no fibre stimulated yet when "
+                                   "no motor unit has fired (config.motor_units_firing_in_5ms)")
     g.close()
 
     mixed = None
@@ -558,6 +574,7 @@ def main(argv=None):
         },
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "only_if_stimulated": only_if_stimulated,
         "workloads": extras,
     }
     print(json.dumps(out), flush=True)

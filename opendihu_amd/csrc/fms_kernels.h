@@ -314,6 +314,29 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   const double *row = sr.row();
   const double t0 = half_start(row, PAIR ? 1 : half), dt = row[2];
   const double t1 = PAIR ? half_start(row + 4, 0) : 0.0, dt1 = PAIR ? row[4 + 2] : 0.0;
+  const int nj = PAIR ? 2 * n0D : n0D;
+  const int spike_step = PAIR ? (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1) : 0;
+  // whether this lane's Heun step j computes (the per-fibre flag, or the
+  // reference's point-buffer rule)
+  auto lane_computes = [&](int j) {
+    const bool second = PAIR && j >= n0D;
+    const int jj = second ? j - n0D : j;
+    const unsigned char *hf = flags + (second ? (long long)fb.n_fibers * n0D : 0LL);
+    return st.vc_width > 0 && st.only_if_stimulated ? vc_compute(st, fb.n_fibers, hf, n0D, i, jj)
+                                                    : (hf[(long long)f * n0D + jj] & FLAG_COMPUTE) != 0;
+  };
+  // a wave none of whose lanes computes a Heun step of this launch (fibres
+  // before their first stimulation under onlyComputeIfHasBeenStimulated) leaves
+  // every state as it is and stores no algebraics: only the spike detector
+  // reads Vm
+  bool any = false;
+  for (int j = 0; j < nj && !any; j++) any = __builtin_amdgcn_ballot_w64(lane_computes(j)) != 0;
+  if (!any) {
+    if constexpr (PAIR) {
+      if (spk.enabled && valid) spike_update(spk, i, vm[i], spike_step);
+    }
+    return;
+  }
   double p[M::NP > 0 ? M::NP : 1];
 #pragma unroll
   for (int k = 0; k < M::NP; k++) p[k] = fb.params[(long long)f * M::NP + k];
@@ -324,14 +347,14 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
   for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * fb.gate_stride + i];
   const double pbase = M::NP > 0 ? p[st.stim_param] : 0.0;
   double alg[ALG ? M::NA : 1] = {};
-  const int nj = PAIR ? 2 * n0D : n0D;
+  bool last_computes = false;   // the last Heun step computes: its algebraics are stored
   for (int j = 0; j < nj; j++) {
     const bool second = PAIR && j >= n0D;
     const int jj = second ? j - n0D : j;
     const unsigned char *hf = flags + (second ? (long long)fb.n_fibers * n0D : 0LL);
     const unsigned char fl = hf[(long long)f * n0D + jj];
-    const bool compute = st.vc_width > 0 && st.only_if_stimulated ? vc_compute(st, fb.n_fibers, hf, n0D, i, jj)
-                                                                  : (fl & FLAG_COMPUTE) != 0;
+    const bool compute = lane_computes(j);
+    last_computes = compute;
     if (__builtin_amdgcn_ballot_w64(compute) != 0) {   // wave-uniform
       const int on = fl & FLAG_STIM;
       const bool clamp = compute && st.mode == 1 && on && inset;
@@ -345,7 +368,7 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
       }
     }
     if constexpr (PAIR) {
-      if (j == n0D - 1 && spk.enabled && valid) spike_update(spk, i, V, (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1));
+      if (j == n0D - 1 && spk.enabled && valid) spike_update(spk, i, V, spike_step);
     }
   }
   if (valid) {
@@ -353,14 +376,17 @@ __global__ __launch_bounds__(256) void k_reaction(DevFibers fb, DevStim st, doub
 #pragma unroll
     for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * fb.gate_stride + i] = g[k];
     if constexpr (ALG) {
-      for (int k = 0; k < n_alg_transfer; k++) {
-        const int a = alg_idx[k];
-        double v = 0;
+      // a node whose last Heun step did not compute keeps the algebraics it had
+      // (the reference calls no generated code for a skipped point buffer)
+      if (last_computes)
+        for (int k = 0; k < n_alg_transfer; k++) {
+          const int a = alg_idx[k];
+          double v = 0;
 #pragma unroll
-        for (int q = 0; q < M::NA; q++)
-          if (q == a) v = alg[q];
-        if (!M::alg_is_param(a)) alg_transfer[(long long)k * N + i] = v;
-      }
+          for (int q = 0; q < M::NA; q++)
+            if (q == a) v = alg[q];
+          if (!M::alg_is_param(a)) alg_transfer[(long long)k * N + i] = v;
+        }
     }
   }
 }
@@ -388,12 +414,31 @@ __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, 
   const double amp = st.mode == 2 ? st.amplitude[f] : 0.0;
   const int nj = PAIR ? 2 * n0D : n0D;
   const int spike_step = PAIR ? (int)(sr.cursor[1] + sr.cursor[0] + sr.q + 1) : 0;
+  const int lead = off & 31;
+  // a fibre none of whose Heun steps in this launch computes (before its first
+  // stimulation under onlyComputeIfHasBeenStimulated) keeps its states: only the
+  // spike detector reads Vm
+  bool any = false;
+  for (int j = 0; j < nj; j++) {
+    const bool second = PAIR && j >= n0D;
+    const int jj = second ? j - n0D : j;
+    any = any || (flags[(second ? (long long)fb.n_fibers * n0D : 0LL) + (long long)f * n0D + jj] & FLAG_COMPUTE);
+  }
+  if (!__builtin_amdgcn_readfirstlane((int)any)) {
+    if constexpr (PAIR) {
+      if (spk.enabled)
+        for (int base = -lead; base < n; base += T) {
+          const int lr = base + (int)threadIdx.x;
+          if (lr >= 0 && lr < n) spike_update(spk, off + lr, vm[off + lr], spike_step);
+        }
+    }
+    return;
+  }
   // passes start on a 32-node boundary of the concatenated arrays (a 256-byte
   // FP64 or 128-byte FP32 segment): lanes before the fibre's first node shadow
   // it, lanes past its end shadow its last node, and shadows store nothing. A
   // fibre's node count plus its offset within the segment rarely needs one more
   // pass (1481 nodes: 24 passes either way).
-  const int lead = off & 31;
   for (int base = -lead; base < n; base += T) {
     const int lr = base + (int)threadIdx.x;
     const bool valid = lr >= 0 && lr < n;

@@ -68,6 +68,43 @@ def _shorten_pair(kernels, thomas, F=6, n=301, steps=1500, precision="fp64", dt=
     return g, o
 
 
+@pytest.mark.parametrize("threads", [64, 256])
+def test_shorten_only_if_stimulated_idle_fibres(threads):
+    """onlyComputeIfHasBeenStimulated with the per-fibre reaction kernel: the
+    workgroup of a fibre none of whose Heun steps computes leaves its states
+    alone (the spike detector still reads Vm); half the motor units never fire,
+    so their fibres stay at their initial states. Bit-exact vs the oracle."""
+    from opendihu_amd.fibers import FiberBatch
+    lib = _lib_or_skip(f"libfms_{SHORTEN}.so")
+    if not os.path.exists(os.path.join(ORACLE_BUILD, f"libfmso_{SHORTEN}.so")):
+        pytest.skip("oracle model not built")
+    F, n, steps, dt = 6, 301, 600, 2e-4
+    mesh, stim = c2(n_fibers=F, n_nodes=n, length_cm=(n - 1) * 0.01, dt=dt, end_time=steps * dt, rate_hz=4000.0)
+    mu = stim["motor_unit"]
+    silent = sorted(set(mu.tolist()))[::2]
+    for m in silent:
+        stim["firing_events"][:, m % stim["firing_events"].shape[1]] = 0
+    g = FiberBatch(**mesh, scheme="IE", kernels="split", thomas="serial", library=lib)
+    g.set_tuning(reaction_per_fibre=1, reaction_threads=threads, pair_halves=1)
+    o = OracleBatch(**mesh, scheme="IE", model=SHORTEN, threads=8)
+    for b in (g, o):
+        b.set_firing_table_stimulation(**stim)
+        b.set_only_compute_if_stimulated(True)
+        b.enable_spike_detection(0.0)
+        b.advance(0.0, dt, steps)
+    for k in range(g.n_states):
+        np.testing.assert_array_equal(g.get_state(k), o.get_state(k), err_msg=f"state {k}")
+    for a, b in zip(g.spikes(), o.spikes()):
+        np.testing.assert_array_equal(a, b)
+    idle = [f for f in range(F) if mu[f] in silent]
+    busy = [f for f in range(F) if mu[f] not in silent]
+    assert idle and busy, (mu, silent)
+    y0 = o.get_state(2)
+    for f in idle:
+        assert np.all(y0[f * n:(f + 1) * n] == y0[idle[0] * n]), "idle fibres keep their initial states"
+    assert (o.spikes()[1] > 0).sum() > 0
+
+
 @pytest.mark.parametrize("kernels", ["split", "split-unpaired", "split-64", "split-64-unpaired", "split-per-node",
                                      "split-per-node-unpaired", "fused"])
 def test_shorten_bit_exact(kernels):

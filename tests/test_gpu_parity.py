@@ -310,6 +310,41 @@ def test_only_compute_if_stimulated(kernels):
     assert 0 < len(untouched) < 12, "some fibres must still be unstimulated at t = 0.8 ms"
 
 
+@pytest.mark.parametrize("kernels,pair", [("split", 1), ("split", 0), ("fused", 1)])
+def test_only_compute_if_stimulated_idle_fibres_spikes_and_algebraics(kernels, pair):
+    """onlyComputeIfHasBeenStimulated with fibres that stay unstimulated: waves
+    and workgroups none of whose Heun steps computes skip their states, the
+    spike detector still follows Vm (changed by diffusion only), and a node whose
+    last Heun step did not compute keeps the algebraics it had (zero before its
+    first computed step), as the reference's skipped point buffers do; fibres of
+    200-600 nodes put stimulated and idle fibres in one wave. Bit-exact vs the
+    oracle over several calls."""
+    mesh, _ = uneven(12, lo=200, hi=600, seed=9)
+    g = _gpu_batch(mesh, scheme="IE", kernels=kernels, thomas="serial")
+    g.set_tuning(pair_halves=pair)
+    o = OracleBatch(**mesh, scheme="IE")
+    for b in (g, o):
+        _firing_setup(b, mesh, width=4)
+        b.set_only_compute_if_stimulated(True)
+        b.enable_spike_detection(-70.0)
+        b.set_algebraics_for_transfer([0, 3, 5])
+    t = 0.0
+    for k in (150, 1, 249):
+        for b in (g, o):
+            b.advance(t, 2e-3, k)
+        t += k * 2e-3
+        np.testing.assert_array_equal(g.get_states(), o.get_states(), err_msg=f"after t = {t}")
+        np.testing.assert_array_equal(g.get_algebraics(), o.algebraics(), err_msg=f"algebraics after t = {t}")
+    for a, b in zip(g.spikes(), o.spikes()):
+        np.testing.assert_array_equal(a, b)
+    off = np.concatenate([[0], np.cumsum(mesh["n_nodes"])])
+    idle = [f for f in range(12) if np.all(o.get_state(1)[off[f]:off[f + 1]] == 0.05)]
+    assert 0 < len(idle) < 12, "some fibres must stay unstimulated"
+    alg = o.algebraics()
+    assert all(np.all(alg[:, off[f]:off[f + 1]] == 0) for f in idle), "idle fibres keep zero algebraics"
+    assert (o.spikes()[1] > 0).sum() > 0, "stimulated fibres must fire"
+
+
 @pytest.mark.parametrize("kernels,n0D", [("split", 1), ("split", 3), ("fused", 1)])
 def test_only_compute_if_stimulated_point_buffer_order(kernels, n0D):
     """onlyComputeIfHasBeenStimulated in the reference's point-buffer order
