@@ -883,6 +883,11 @@ int fms_equilibrium_states(double *states, const double *params, double dt, long
   if (iterations) *iterations = it;
   if (max_increment) *max_increment = incr;
   if (dt_out) *dt_out = dt;
+  // a state that became NaN never raises the maximum increment (fabs(NaN) > x
+  // is false), so the search could stop "converged" on it: report it instead
+  for (int k = 0; k < NS; k++)
+    if (!std::isfinite(u[k].v))
+      return fail(FMS_ERR_STATE, "the equilibrium search reached a non-finite value of state " + std::to_string(k));
   if (!(incr < 1e-5))
     return fail(FMS_ERR_STATE, "equilibrium values of the states were not found within " +
                                    std::to_string(max_iterations) + " RK4 iterations (last increment " +

@@ -119,6 +119,23 @@ def cases_steady_state(v):
     return np.array(hh_steady_state_gates(v))
 
 
+def test_equilibrium_search_reports_non_finite_states():
+    """a state that turns NaN never raises the search's maximum increment; the
+    library reports it instead of returning the NaN states as converged"""
+    import ctypes
+    from opendihu_amd import _lib as L
+    lib = L.load_library("libfms_hodgkin_huxley_1952.so")
+    ns, _, npar, _ = L.model_info(lib)
+    y, p0 = np.zeros(ns), np.zeros(max(npar, 1))
+    L.check(lib, lib.fms_model_defaults(L.as_ptr(y, L._dp), L.as_ptr(p0, L._dp)), "defaults")
+    y[2] = np.nan   # h: after one RK4 step every state is NaN and every |increment| compares false
+    it, inc, dt_out = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+    rc = lib.fms_equilibrium_states(L.as_ptr(y, L._dp), L.as_ptr(p0, L._dp), 1e-4, 1000, ctypes.byref(it),
+                                    ctypes.byref(inc), ctypes.byref(dt_out))
+    assert rc == L.FMS_ERR_STATE
+    assert "non-finite" in L.last_error(lib)
+
+
 def test_front_end_equilibrium_output_and_transfer(tmp_path):
     """initializeStatesToEquilibrium, statesForTransfer / algebraicsForTransfer
     and PythonFile output through the settings front end: the run starts from
