@@ -33,7 +33,8 @@ boundary conditions of the nested solvers) are accepted silently; every other ke
 this front end does not apply raises a warning that names it.
 Extension keys (ours): "vcVectorWidth" (stimulated lanes of the vc path, default 4),
 "neuromuscularJunctionSeed" (the reference draws NMJ positions unseeded),
-"kernels" ("fused" | "split"), "thomas" ("partitioned" | "serial"), "precision"
+"kernels" ("auto": the autotuner's fastest bit-identical schedule, timed on the run's first
+split steps at initialisation | "fused" | "split"), "thomas" ("partitioned" | "serial"), "precision"
 ("fp64" | "mixed"), "pointBufferOrder" (True: onlyComputeIfHasBeenStimulated in the
 reference's sweep over buffers of vcVectorWidth nodes, compute.tpp:131-216).
 """
@@ -199,7 +200,7 @@ class FastMonodomainSolver:
             precision=precision,
             exp_mode="approx" if cellml0.get("approximateExponentialFunction", True) else "libm",
             thomas=rc.get("thomas", "partitioned"),
-            kernels=rc.get("kernels", "fused"),
+            kernels="fused" if rc.get("kernels", "auto") == "auto" else rc["kernels"],
             value_for_stimulated_point=float(rc.get("valueForStimulatedPoint", 20.0)),
             library=library,
         )
@@ -267,6 +268,23 @@ class FastMonodomainSolver:
         if rc.get("disableComputationWhenStatesAreCloseToEquilibrium", True):
             warnings.warn("disableComputationWhenStatesAreCloseToEquilibrium is not applied: every node is "
                           "computed every step (set it to False to silence this warning)", stacklevel=2)
+        if rc.get("pointBufferOrder", False):
+            nmj_ = self.stim_nmj if self.stim is not None else self.n_nodes // 2
+            self.batch.set_point_buffer_order(int(rc.get("vcVectorWidth", 4)), nmj_)
+
+        # time stepping
+        sched = L.schedule(self.batch.lib, float(rc["endTime"]), float(rc["timeStepWidth"]),
+                           float(strang0.get("endTime", rc["timeStepWidth"])), float(strang0["timeStepWidth"]),
+                           0.5 * float(strang0["timeStepWidth"]), float(heun0["timeStepWidth"]),
+                           float(strang0["timeStepWidth"]), float(diff0["timeStepWidth"]))
+        self.schedule = sched
+        self.tuned_schedule = None
+        if rc.get("kernels", "auto") == "auto" and not rc.get("pointBufferOrder", False):
+            # the candidates give the same bits; the trial restores every state
+            rows = self._step_rows()
+            if len(rows):
+                self.tuned_schedule = self.batch.autotune(rows[:64], n0D=int(sched[0, 4]), allow_split=True)
+
         alg = _list(cellml0.get("algebraicsForTransfer", []))
         if alg:
             self.batch.set_algebraics_for_transfer(alg)
@@ -274,9 +292,6 @@ class FastMonodomainSolver:
         self.states_for_transfer = _list(cellml0.get("statesForTransfer", [0])) or [0]
         if any(not 0 <= k < n_states for k in self.states_for_transfer):
             raise ValueError(f"statesForTransfer {self.states_for_transfer}: state numbers are 0..{n_states - 1}")
-        if rc.get("pointBufferOrder", False):
-            nmj_ = self.stim_nmj if self.stim is not None else self.n_nodes // 2
-            self.batch.set_point_buffer_order(int(rc.get("vcVectorWidth", 4)), nmj_)
         self.output_writers = [w for w in _list(strang0.get("OutputWriter", [])) if isinstance(w, dict)]
         for w in self.output_writers:
             if w.get("format") != "PythonFile":
@@ -285,13 +300,6 @@ class FastMonodomainSolver:
         self.output_writers = [w for w in self.output_writers if w.get("format") == "PythonFile"]
         self.stimulation_log_file = cellml0.get("stimulationLogFilename", "out/stimulation.log")
 
-        # time stepping
-        from . import _lib as L
-        sched = L.schedule(self.batch.lib, float(rc["endTime"]), float(rc["timeStepWidth"]),
-                           float(strang0.get("endTime", rc["timeStepWidth"])), float(strang0["timeStepWidth"]),
-                           0.5 * float(strang0["timeStepWidth"]), float(heun0["timeStepWidth"]),
-                           float(strang0["timeStepWidth"]), float(diff0["timeStepWidth"]))
-        self.schedule = sched
         self.initialized = True
 
     def _write_equilibrium_file(self, stem, eq):
@@ -336,11 +344,7 @@ class FastMonodomainSolver:
         n0 = np.unique(s[:, 4])
         if len(n0) != 1:
             raise ValueError("the number of Heun steps per half step changes between calls")
-        rows = []
-        for t0, dts, ns, dt0, _, dt1, _ in s:
-            for k in range(int(ns)):
-                rows.append((t0 + k * dts, dts, dt0, dt1))
-        rows = np.array(rows)
+        rows = self._step_rows()
         writers = self._writers()
         if not writers:
             self.batch.advance_schedule(rows, int(n0[0]))
@@ -358,6 +362,12 @@ class FastMonodomainSolver:
                         w.write(self, step=step, time=float(chunk[-1, 0] + chunk[-1, 1]))
         self.batch.synchronize()
         self.current_time = float(s[-1, 0] + s[-1, 1] * s[-1, 2])
+
+    def _step_rows(self) -> np.ndarray:
+        """one record (t, dt_split, dt0D, dt1D) per split step of all RepeatedCall calls"""
+        rows = [(t0 + k * dts, dts, dt0, dt1) for t0, dts, ns, dt0, _, dt1, _ in self.schedule
+                for k in range(int(ns))]
+        return np.array(rows, np.float64).reshape(-1, 4)
 
     def _writers(self):
         from .output import PythonFileWriter

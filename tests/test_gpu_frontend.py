@@ -56,6 +56,8 @@ def test_settings_front_end_matches_oracle(only_if):
     s = FastMonodomainSolver(cfg)
     s.initialize()
     assert np.all(s.schedule[:, 4] == 5) and s.schedule.shape[0] == 2000
+    # "kernels" defaults to "auto": the autotuner chose among bit-identical schedules
+    assert s.tuned_schedule is not None and s.tuned_schedule["rejected"] == 0, s.tuned_schedule
     s.advance_time_span()
     n = s.n_nodes
     off = np.concatenate([[0], np.cumsum(n)[:-1]])
