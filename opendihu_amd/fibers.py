@@ -176,8 +176,9 @@ class FiberBatch:
         two reaction halves between two solves in one launch (1, default) or one
         launch per half (0); threads per fibre of the per-fibre reaction (256 or 64).
         Adjoint backward: segment recompute beside the reverse sweep (0), the same
-        with the reverse sweep's priority raised (1), or one after the other (2);
-        threads per fibre workgroup of the reverse sweep (256, 512 or 1024)."""
+        with the reverse sweep's priority raised (1) or the recompute's (3), or one
+        after the other (2); threads per fibre workgroup of the reverse sweep (256,
+        512 or 1024)."""
         for key, value in ((L.FMS_TUNE_STEPS_PER_LAUNCH, steps_per_launch), (L.FMS_TUNE_RESIDENT_GATES, resident_gates),
                            (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps),
                            (L.FMS_TUNE_REACTION_PER_FIBRE, reaction_per_fibre),
@@ -187,7 +188,8 @@ class FiberBatch:
                 self._chk(self.lib.fms_set_tuning(self.handle, key, int(value)), "fms_set_tuning")
 
     def profile_kernels(self, on: bool = True) -> None:
-        """time every kernel launch with HIP events (for measurement runs)"""
+        """time every kernel launch with a start/stop event pair bound to it (for
+        measurement runs; kernel_times() reads the sums back)"""
         self._chk(self.lib.fms_set_tuning(self.handle, L.FMS_TUNE_PROFILE_KERNELS, int(bool(on))), "fms_set_tuning")
 
     def kernel_times(self, reset: bool = False) -> dict:
