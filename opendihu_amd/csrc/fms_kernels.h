@@ -404,7 +404,6 @@ __global__ __launch_bounds__(T) void k_reaction_fibre(DevFibers fb, DevStim st, 
                                                       StepRef sr, int half, int n0D, double vstim,
                                                       const int *__restrict__ order, DevSpikes spk) {
   const int f = order ? order[blockIdx.x] : (int)blockIdx.x;
-  const long long N = fb.n_total;
   const int off = fb.fiber_off[f];
   const int n = fb.fiber_off[f + 1] - off;
   const double *row = sr.row();

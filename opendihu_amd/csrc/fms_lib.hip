@@ -1655,7 +1655,6 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
     return fail(FMS_ERR_UNSUPPORTED, "the Vm clamp of firing-table stimulation is not differentiable; use current pulses");
   if (s->only_if_stimulated) return fail(FMS_ERR_UNSUPPORTED, "onlyComputeIfHasBeenStimulated is not differentiable");
   if (s->nmax > 2048) return fail(FMS_ERR_UNSUPPORTED, "the adjoint takes fibres of <= 2048 nodes");
-  const long long N = s->N;
   const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
   const int nseg = (K + Kc - 1) / Kc;
   int rc;
