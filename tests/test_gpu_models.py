@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-from cases import c1, c2, uneven
+from cases import c2, uneven
 from oracle_binding import OracleBatch, ORACLE_BUILD
 
 pytestmark = pytest.mark.gpu
@@ -166,7 +166,6 @@ def test_shorten_mixed_precision():
 
 
 def _random_points(lib_name, oracle_model, n=4000, seed=0):
-    import ctypes
     from opendihu_amd import _lib as L
     from oracle_binding import oracle_library, P, _dp
     g = L.load_library(lib_name)

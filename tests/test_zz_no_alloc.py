@@ -2,8 +2,6 @@
 device scratch array) against fms_advance_schedule on the same inputs: two calls
 in a row, the second with fewer steps, bit-identical states, spikes and
 stimulation log; a call larger than the scratch is refused with an error."""
-import ctypes
-
 import numpy as np
 import pytest
 
