@@ -270,8 +270,10 @@ def bench_c3(cpu=True):
 
 def bench_c4():
     """Adjoint: HH on 256 fibres x 1000 nodes, 5 ms at dt=1e-3 ms (5000 split
-    steps), 32 sensors per fibre, checkpoint every 100 steps. Cost of the
-    gradient (recording forward + backward) against one plain forward solve,
+    steps), 32 sensors per fibre. Cost of the gradient (recording forward +
+    backward) against one plain forward solve, with the whole tape in device
+    memory (92 GB: the forward records it, the backward only reverses) and with
+    a checkpoint every 100 steps (the backward recomputes each segment's tape),
     and one directional central difference (h=1e-6) at full size."""
     from cases import uneven, AM, CM
     from opendihu_amd.fibers import FiberBatch
@@ -291,8 +293,10 @@ def bench_c4():
     dev = torch.device("cuda")
     target = prob.forward_tensor(torch.tensor(theta_true, device=dev))
     th0 = torch.tensor(theta0, device=dev)
-    # warm-up of every path (code objects, buffers); the adjoint schedule (segment
-    # recompute beside the reverse sweep or before it) timed and chosen here
+    # warm-up of every path (code objects, buffers); the reverse sweep's
+    # workgroup size (and with checkpoints the recompute's schedule) timed and
+    # chosen here
+    prob.set_tape("whole")
     prob.backward_tensor(prob.forward_tensor(th0) - target)
     tuned = prob.tune_backward(theta0, (prob.forward(theta0) - target.cpu().numpy()))
     # each phase three times, the fastest counted (all are reported)
@@ -309,18 +313,28 @@ def bench_c4():
         b.synchronize()
         plain.append(time.perf_counter() - w0)
     # gradient: recording forward + backward, traces and adjoints in device memory
-    fwd, bwd = [], []
-    for _ in range(reps):
-        w0 = time.perf_counter()
-        tr = prob.forward_tensor(th0)
-        torch.cuda.synchronize()
-        fwd.append(time.perf_counter() - w0)
-        resid = tr - target
-        torch.cuda.synchronize()
-        w0 = time.perf_counter()
-        grad = prob.backward_tensor(resid).cpu().numpy()
-        bwd.append(time.perf_counter() - w0)
+    def gradient_times():
+        fwd, bwd = [], []
+        for _ in range(reps):
+            w0 = time.perf_counter()
+            tr = prob.forward_tensor(th0)
+            torch.cuda.synchronize()
+            fwd.append(time.perf_counter() - w0)
+            resid = tr - target
+            torch.cuda.synchronize()
+            w0 = time.perf_counter()
+            grad = prob.backward_tensor(resid).cpu().numpy()
+            bwd.append(time.perf_counter() - w0)
+        return fwd, bwd, grad
+    fwd, bwd, grad = gradient_times()
+    assert prob.whole_tape()
     t_plain, t_fwd, t_bwd = min(plain), min(fwd), min(bwd)
+    # the same with checkpoints: the backward recomputes each segment
+    prob.set_tape("segments")
+    tuned_ck = prob.tune_backward(theta0, (prob.forward(theta0) - target.cpu().numpy()))
+    fwd_ck, bwd_ck, grad_ck = gradient_times()
+    assert not prob.whole_tape()
+    prob.set_tape("whole")
     loss = lambda th: 0.5 * float(((prob.forward_tensor(torch.tensor(th, device=dev)) - target) ** 2).sum())  # noqa: E731
     v = np.random.default_rng(43).standard_normal(theta0.shape) * theta0
     h = 1e-6
@@ -329,11 +343,16 @@ def bench_c4():
     fdr = (4 * fd - fd2) / 3
     ad = float(np.sum(grad * v))
     b.close()
-    return dict(workload="hh_adjoint_256x1000_CN_dt1e-3_5ms_32sensors_ckpt100", steps=n_steps,
+    ck = dict(recording_forward_s=min(fwd_ck), backward_s=min(bwd_ck),
+              repetitions_s=dict(recording_forward=fwd_ck, backward=bwd_ck),
+              gradient_cost_vs_forward=(min(fwd_ck) + min(bwd_ck)) / t_plain, adjoint_schedule=tuned_ck,
+              max_abs_grad_difference_vs_whole_tape=float(np.max(np.abs(grad_ck - grad))) if
+              tuned_ck["threads"] == tuned["threads"] else None)
+    return dict(workload="hh_adjoint_256x1000_CN_dt1e-3_5ms_32sensors", steps=n_steps, tape="whole (92 GB)",
                 plain_forward_s=t_plain, recording_forward_s=t_fwd, backward_s=t_bwd,
                 repetitions_s=dict(plain_forward=plain, recording_forward=fwd, backward=bwd),
                 gradient_cost_vs_forward=(t_fwd + t_bwd) / t_plain, backward_vs_forward=t_bwd / t_plain,
-                adjoint_schedule=tuned,
+                adjoint_schedule=tuned, checkpoints_every_100_steps=ck,
                 directional_derivative=dict(adjoint=ad, central_fd=fd, h=h, rel_err=abs(ad - fd) / abs(fd),
                                             central_fd_2h=fd2, richardson=fdr, rel_err_richardson=abs(ad - fdr) / abs(fdr)),
                 note="wall-clock per call, fastest of three; traces and their adjoints stay in device memory (torch tensors)")

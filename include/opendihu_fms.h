@@ -228,6 +228,20 @@ int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [
    gradient is summed per wave and then over the waves, so its last bits depend
    on this number. */
 #define FMS_TUNE_ADJOINT_THREADS 11
+/* where the adjoint backward finds the forward's tape (9 doubles per node and
+   split step). FMS_ADJOINT_TAPE_SEGMENTS: fms_adjoint_forward keeps a
+   checkpoint every checkpoint_interval steps and the backward recomputes each
+   segment (two segment tapes in device memory, FMS_TUNE_ADJOINT schedules the
+   recompute). FMS_ADJOINT_TAPE_WHOLE: fms_adjoint_forward records the whole
+   tape (72 B per node and step, 92 GB for 256 x 1000 nodes over 5000 steps) and
+   the backward only reverses it. FMS_ADJOINT_TAPE_AUTO (default): the whole
+   tape when it fits in half of the device memory free at the forward, else
+   segments. The mode of the last forward decides the backward's. Same
+   arithmetic, same gradients. */
+#define FMS_TUNE_ADJOINT_TAPE 12
+#define FMS_ADJOINT_TAPE_AUTO (-1)
+#define FMS_ADJOINT_TAPE_SEGMENTS 0
+#define FMS_ADJOINT_TAPE_WHOLE 1
 #define FMS_TUNE_REACTION_THREADS 9   /* split kernels, models of > 8 states, reaction per fibre: threads per fibre
                                          workgroup, 256 (default) or 64 (one wave: four fibres share a CU and their
                                          loads do not coincide). Same arithmetic. */
@@ -295,16 +309,20 @@ int fms_set_sensors(FmsSolver *solver, const int *sensor_nodes, int n_sensors);
 /* Discrete adjoint (Hodgkin-Huxley model, FP64, current-pulse or no stimulation,
    fibres of <= 2048 nodes; no counterpart in the reference).
    fms_adjoint_forward runs n_steps split steps from the current state
-   (steps[k*4 + {0..3}] = {t, dt_split, dt0D, dt1D}), stores the state every
-   checkpoint_interval steps and returns the sensor traces
-   traces[(k*n_fibers + f)*n_sensors + j] (host buffer, may be NULL).
+   (steps[k*4 + {0..3}] = {t, dt_split, dt0D, dt1D}), records the whole tape or
+   stores the state every checkpoint_interval steps (FMS_TUNE_ADJOINT_TAPE) and
+   returns the sensor traces traces[(k*n_fibers + f)*n_sensors + j] (host
+   buffer, may be NULL).
    fms_adjoint_backward takes dL/dtraces in the same layout and returns
    grad_params[f*3 + {0,1,2}] = dL/d(g_Na, g_K, g_L) and grad_prefactor[f] = dL/dprefactor
-   (dL/dsigma = grad_prefactor / (Am*Cm)). It recomputes each checkpoint segment
-   with a tape and leaves the solver's states unspecified. */
+   (dL/dsigma = grad_prefactor / (Am*Cm)). With checkpoints it recomputes each
+   segment with a tape first; it leaves the solver's states unspecified. */
 int fms_adjoint_forward(FmsSolver *solver, const double *steps, int n_steps, int n0D, int checkpoint_interval,
                         double *traces);
 int fms_adjoint_backward(FmsSolver *solver, const double *dtraces, double *grad_params, double *grad_prefactor);
+/* *whole = 1 if the last fms_adjoint_forward recorded the whole tape, 0 if it
+   kept checkpoints */
+int fms_adjoint_whole_tape(FmsSolver *solver, int *whole);
 /* the same with the traces, their adjoints and the gradients in device memory
    (e.g. torch tensors on the GPU): no host copies */
 int fms_adjoint_forward_device(FmsSolver *solver, const double *steps, int n_steps, int n0D,

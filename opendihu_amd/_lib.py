@@ -25,6 +25,8 @@ FMS_TUNE_PAIR_HALVES, FMS_TUNE_REACTION_THREADS, FMS_TUNE_ADJOINT = 8, 9, 10
 FMS_ADJOINT_OVERLAP, FMS_ADJOINT_OVERLAP_PRIORITY, FMS_ADJOINT_SERIAL = 0, 1, 2
 FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY = 3
 FMS_TUNE_ADJOINT_THREADS = 11
+FMS_TUNE_ADJOINT_TAPE = 12
+FMS_ADJOINT_TAPE_AUTO, FMS_ADJOINT_TAPE_SEGMENTS, FMS_ADJOINT_TAPE_WHOLE = -1, 0, 1
 FMS_KERNEL_CLASSES = ("reaction", "diffusion", "fused", "other", "reaction_pair")
 
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -115,6 +117,7 @@ EXPORTED_SYMBOLS = [
     "fms_reference_set_dims", "initializeArrays", "computeMonodomain", "fms_graph_counts", "fms_autotune",
     "fms_eval_rhs", "fms_eval_heun", "fms_adjoint_forward_device", "fms_adjoint_backward_device",
     "fms_kernel_times", "fms_set_point_buffer_order", "fms_equilibrium_states", "fms_advance_schedule_noalloc",
+    "fms_adjoint_whole_tape",
 ]
 
 _LIBS: dict[str, ctypes.CDLL] = {}
@@ -181,6 +184,7 @@ def load_library(name: str = HH_LIBRARY) -> ctypes.CDLL:
     lib.fms_adjoint_backward.argtypes = [v, _dp, _dp, _dp]
     lib.fms_adjoint_forward_device.argtypes = [v, _dp, i, i, i, v]
     lib.fms_adjoint_backward_device.argtypes = [v, v, v, v]
+    lib.fms_adjoint_whole_tape.argtypes = [v, _ip]
     lib.fms_eval_rhs.argtypes = [_dp, _dp, d, i, i, _dp, _dp]
     lib.fms_eval_heun.argtypes = [_dp, _dp, d, d, i, i, i]
     lib.fms_kernel_times.argtypes = [v, _dp, ctypes.POINTER(ctypes.c_longlong), i]

@@ -7,7 +7,10 @@ conductivity, through the hand-written discrete adjoint of the split step
 
 ``problem`` (:class:`AdjointProblem`) holds the fibre batch (Hodgkin-Huxley,
 current-pulse stimulation), the initial states, the split-step table, the
-sensor nodes and the checkpoint interval. Forward and backward run on the GPU;
+sensor nodes, the checkpoint interval and where the tape lives: the forward
+records the whole tape in device memory (72 B per node and split step) and the
+backward only reverses it, or the forward keeps a checkpoint per interval and
+the backward recomputes each segment's tape. Forward and backward run on the GPU;
 with theta on the GPU the traces and their adjoints stay in device memory, with
 theta on the CPU they are CPU float64 tensors.
 """
@@ -22,7 +25,14 @@ from .fibers import FiberBatch
 
 
 class AdjointProblem:
-    def __init__(self, batch: FiberBatch, steps, sensor_nodes, checkpoint_interval=100, n0D=1, am=500.0, cm=1.0):
+    TAPES = {"auto": L.FMS_ADJOINT_TAPE_AUTO, "segments": L.FMS_ADJOINT_TAPE_SEGMENTS,
+             "whole": L.FMS_ADJOINT_TAPE_WHOLE}
+
+    def __init__(self, batch: FiberBatch, steps, sensor_nodes, checkpoint_interval=100, n0D=1, am=500.0, cm=1.0,
+                 tape: str = "auto"):
+        """tape: "whole" (the forward records the whole tape), "segments"
+        (checkpoints; the backward recomputes each segment) or "auto" (the whole
+        tape when it fits in half of the free device memory)"""
         if batch.model_name != "hodgkin_huxley_1952":
             raise ValueError("the adjoint is implemented for the Hodgkin-Huxley model")
         self.batch = batch
@@ -38,6 +48,7 @@ class AdjointProblem:
         # forwards run so far: the solver keeps the checkpoints, step table and
         # parameters of the last one only, which the backward reverses
         self.generation = 0
+        self.set_tape(tape)
         lib = batch.lib
         L.check(lib, lib.fms_set_sensors(batch.handle, L.as_ptr(self.sensor_nodes, L._ip), self.n_sensors),
                 "fms_set_sensors")
@@ -97,21 +108,39 @@ class AdjointProblem:
                 "fms_adjoint_backward_device")
         return torch.cat([gp, (gpf / self.am_cm)[:, None]], dim=1)
 
+    def set_tape(self, tape: str) -> None:
+        """where the next forward keeps the tape ("whole", "segments" or "auto")"""
+        if tape not in self.TAPES:
+            raise ValueError(f"tape is one of {sorted(self.TAPES)}")
+        self.batch.set_tuning(adjoint_tape=self.TAPES[tape])
+
+    def whole_tape(self) -> bool:
+        """whether the last forward recorded the whole tape (else checkpoints)"""
+        w = ctypes.c_int(0)
+        L.check(self.batch.lib, self.batch.lib.fms_adjoint_whole_tape(self.batch.handle, ctypes.byref(w)),
+                "fms_adjoint_whole_tape")
+        return bool(w.value)
+
     def tune_backward(self, theta, dtraces, threads=(256, 512)) -> dict:
-        """time fms_adjoint_backward under each adjoint schedule (segment
-        recompute before the reverse sweep, or beside it with equal priority,
-        the sweep's or the recompute's priority raised) and each workgroup size
-        of the reverse sweep, for one forward at theta, and keep the fastest.
-        The schedules give the same gradient; the workgroup size changes the
-        order of the per-fibre prefactor-gradient sum (last bits)."""
+        """time fms_adjoint_backward for one forward at theta under each
+        workgroup size of the reverse sweep and, with checkpoints, each schedule
+        of the segment recompute (before the reverse sweep, or beside it with
+        equal priority, the sweep's or the recompute's priority raised), and
+        keep the fastest. The schedules give the same gradient; the workgroup
+        size changes the order of the per-fibre prefactor-gradient sum (last
+        bits)."""
         import time
         from . import _lib
         times = {}
         self.forward(theta)
+        whole = self.whole_tape()
         names = {0: "overlap", 1: "overlap_priority", 2: "serial", 3: "overlap_recompute_priority"}
+        if whole:
+            names = {2: "whole_tape"}
         for tt in threads:
-            for mode in (_lib.FMS_ADJOINT_SERIAL, _lib.FMS_ADJOINT_OVERLAP, _lib.FMS_ADJOINT_OVERLAP_PRIORITY,
-                         _lib.FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY):
+            for mode in ((_lib.FMS_ADJOINT_SERIAL,) if whole else
+                         (_lib.FMS_ADJOINT_SERIAL, _lib.FMS_ADJOINT_OVERLAP, _lib.FMS_ADJOINT_OVERLAP_PRIORITY,
+                          _lib.FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY)):
                 self.batch.set_tuning(adjoint=mode, adjoint_threads=tt)
                 self.backward(dtraces)   # untimed: buffers, code objects
                 t0 = time.perf_counter()
@@ -119,7 +148,7 @@ class AdjointProblem:
                 times[(tt, mode)] = time.perf_counter() - t0
         best = min(times, key=times.get)
         self.batch.set_tuning(adjoint_threads=best[0], adjoint=best[1])
-        return dict(schedule=names[best[1]], threads=best[0],
+        return dict(schedule=names[best[1]], threads=best[0], tape="whole" if whole else "segments",
                     seconds={f"{names[m]}/{tt}": v for (tt, m), v in times.items()})
 
     def backward(self, dtraces) -> np.ndarray:

@@ -160,6 +160,7 @@ struct FmsSolver {
   int tune_reaction_threads = 256;   // split kernels, per-fibre reaction: threads per fibre workgroup (64 or 256)
   int tune_adjoint = FMS_ADJOINT_SERIAL;    // adjoint backward: recompute beside the reverse sweep or before it
   int tune_adjoint_threads = 512;           // threads per fibre workgroup of the reverse sweep (256, 512, 1024)
+  int tune_adjoint_tape = FMS_ADJOINT_TAPE_AUTO;   // the forward records the whole tape, or checkpoints
 
   // per-kernel timing (fms_set_tuning(FMS_TUNE_PROFILE_KERNELS)): a start and
   // a stop event bound to each launch (hipExtLaunchKernel: the events take the
@@ -185,7 +186,8 @@ struct FmsSolver {
   hipEvent_t events[8] = {};
   long long n_launch_fused = 0, n_launch_reaction = 0, n_launch_thomas = 0;
 
-  // sensors, traces and the adjoint (checkpoints every ckpt_interval steps, tape of one segment)
+  // sensors, traces and the adjoint (checkpoints every ckpt_interval steps and a
+  // tape of two segments, or the whole forward's tape)
   int n_sensors = 0;
   int *d_sensor_nodes = nullptr;   // [F][S] fibre-local
   int *d_sensor_slot = nullptr;    // [N]
@@ -196,6 +198,7 @@ struct FmsSolver {
   long long ckpt_cap = 0, tape_cap = 0;
   std::vector<double> adj_steps;   // host copy of the last adjoint forward's step table
   int adj_n0D = 1, adj_interval = 0, adj_K = 0;
+  int adj_whole = 0;   // the last adjoint forward recorded the whole tape
 
   // algebraics for transfer
   int *d_alg_idx = nullptr;
@@ -1388,6 +1391,11 @@ int fms_set_tuning(FmsSolver *s, int key, int value) {
         return fail(FMS_ERR_UNSUPPORTED, "512 (1024) adjoint threads per fibre take fibres of <= 2048 nodes");
       s->tune_adjoint_threads = value;
       return 0;
+    case FMS_TUNE_ADJOINT_TAPE:
+      if (value < FMS_ADJOINT_TAPE_AUTO || value > FMS_ADJOINT_TAPE_WHOLE)
+        return fail(FMS_ERR_ARGUMENT, "adjoint tape is FMS_ADJOINT_TAPE_AUTO, _SEGMENTS or _WHOLE");
+      s->tune_adjoint_tape = value;
+      return 0;
     case FMS_TUNE_PROFILE_KERNELS:
       if (value < 0 || value > 1) return fail(FMS_ERR_ARGUMENT, "kernel profiling is 0 or 1");
       for (hipEvent_t &e : s->prof_ev)
@@ -1546,19 +1554,26 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     HIP_TRY(hipMemsetAsync(s->d_lam, 0, 4 * N * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->d_gacc, 0, 3 * N * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->d_pfbar, 0, F * sizeof(double), s->stream));
-    // two tape buffers: the recompute of segment seg-1 (stream2) runs beside the
-    // reverse of segment seg (stream)
+    // the whole forward's tape (segment seg at seg * tape_len), or two segment
+    // buffers: the recompute of segment seg-1 (stream2) runs beside the reverse
+    // of segment seg (stream)
     const size_t tape_len = (size_t)Kc * (2 * Model::NS + 1) * N;
-    if ((rc = ensure(&s->d_tape, s->tape_cap, 2 * (long long)tape_len))) return rc;
+    const int nseg = (K + Kc - 1) / Kc;
+    const bool whole = s->adj_whole != 0;
+    if (whole && s->tape_cap < (long long)nseg * (long long)tape_len)
+      return fail(FMS_ERR_STATE, "the adjoint forward's tape is gone; run fms_adjoint_forward again");
+    if (!whole && (rc = ensure(&s->d_tape, s->tape_cap, 2 * (long long)tape_len))) return rc;
     if ((rc = upload_steps(s, s->adj_steps.data(), K))) return rc;
-    if (!s->stream2 && hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess)
-      return fail(FMS_ERR_HIP, "hipStreamCreate failed");
-    for (hipEvent_t &e : s->adj_ev)
-      if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-        return fail(FMS_ERR_HIP, "hipEventCreate failed");
-    // the recompute stream starts after everything enqueued so far (step table, clears)
-    HIP_TRY(hipEventRecord(s->adj_ev[2], s->stream));
-    HIP_TRY(hipStreamWaitEvent(s->stream2, s->adj_ev[2], 0));
+    if (!whole) {
+      if (!s->stream2 && hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess)
+        return fail(FMS_ERR_HIP, "hipStreamCreate failed");
+      for (hipEvent_t &e : s->adj_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+          return fail(FMS_ERR_HIP, "hipEventCreate failed");
+      // the recompute stream starts after everything enqueued so far (step table, clears)
+      HIP_TRY(hipEventRecord(s->adj_ev[2], s->stream));
+      HIP_TRY(hipStreamWaitEvent(s->stream2, s->adj_ev[2], 0));
+    }
     const int saved_spikes = s->spikes;
     s->spikes = 0;
     // a recompute beside the reverse sweep streams its gates: the resident-gate
@@ -1566,37 +1581,38 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     // reverse wave, the streamed kernel's (126) do; both give the same bits
     const int saved_resident = s->tune_resident;
     if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->tune_resident = 0;
-    const int nseg = (K + Kc - 1) / Kc;
     constexpr int T = 256;
     const size_t lds = (size_t)(fms::pcr_lds_doubles<1024>() + 3 * s->nmax) * sizeof(double);
     hipStream_t main_stream = s->stream;
     for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
       const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
       const int buf = idx & 1;
-      double *tape = s->d_tape + buf * tape_len;
-      // recompute the segment from its checkpoint on stream2, recording the tape;
-      // the buffer was last read by the reverse of segment seg+2 (serialised
-      // schedule: on the solver's own stream, before the reverse sweep)
-      if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->stream = s->stream2;
-      if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess)
-        rc = fail(FMS_ERR_HIP, "hipStreamWaitEvent failed");
-      if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), false);
-      fms::DevTrace tr{};
-      tr.tape = tape;
-      tr.raise_priority = s->tune_adjoint == FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY;
-      const long long save = s->step_counter;
-      if (!rc) rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
-      s->step_counter = save;
-      if (!rc && hipEventRecord(s->adj_ev[2 + buf], s->stream) != hipSuccess) rc = FMS_ERR_HIP;
-      s->stream = main_stream;
-      if (rc) break;
-      // reverse the segment on the solver's stream once its tape is recorded
-      if (hipStreamWaitEvent(s->stream, s->adj_ev[2 + buf], 0) != hipSuccess) {
-        rc = fail(FMS_ERR_HIP, "hipStreamWaitEvent failed");
-        break;
+      double *tape = s->d_tape + (whole ? (size_t)seg : (size_t)buf) * tape_len;
+      if (!whole) {
+        // recompute the segment from its checkpoint on stream2, recording the tape;
+        // the buffer was last read by the reverse of segment seg+2 (serialised
+        // schedule: on the solver's own stream, before the reverse sweep)
+        if (s->tune_adjoint != FMS_ADJOINT_SERIAL) s->stream = s->stream2;
+        if (idx >= 2 && hipStreamWaitEvent(s->stream2, s->adj_ev[buf], 0) != hipSuccess)
+          rc = fail(FMS_ERR_HIP, "hipStreamWaitEvent failed");
+        if (!rc) rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), false);
+        fms::DevTrace tr{};
+        tr.tape = tape;
+        tr.raise_priority = s->tune_adjoint == FMS_ADJOINT_OVERLAP_RECOMPUTE_PRIORITY;
+        const long long save = s->step_counter;
+        if (!rc) rc = launch_fused<double, E>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
+        s->step_counter = save;
+        if (!rc && hipEventRecord(s->adj_ev[2 + buf], s->stream) != hipSuccess) rc = FMS_ERR_HIP;
+        s->stream = main_stream;
+        if (rc) break;
+        // reverse the segment on the solver's stream once its tape is recorded
+        if (hipStreamWaitEvent(s->stream, s->adj_ev[2 + buf], 0) != hipSuccess) {
+          rc = fail(FMS_ERR_HIP, "hipStreamWaitEvent failed");
+          break;
+        }
       }
       const double *dtr = d_dtraces + (size_t)k0 * F * S;
-      const int prio = s->tune_adjoint == FMS_ADJOINT_OVERLAP_PRIORITY;
+      const int prio = !whole && s->tune_adjoint == FMS_ADJOINT_OVERLAP_PRIORITY;
       const int at = s->tune_adjoint_threads;
 #define FMS_ADJ_LAUNCH(TT, NPT_, MMAX_)                                                                           \
   hipLaunchKernelGGL((fms::k_adj_fused<E, TT, NPT_, MMAX_>), dim3(F), dim3(TT), lds, s->stream, s->fibers(),       \
@@ -1623,13 +1639,13 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
         rc = fail(FMS_ERR_HIP, "adjoint kernel launch failed");
         break;
       }
-      if (hipEventRecord(s->adj_ev[buf], s->stream) != hipSuccess) {
+      if (!whole && hipEventRecord(s->adj_ev[buf], s->stream) != hipSuccess) {
         rc = fail(FMS_ERR_HIP, "hipEventRecord failed");
         break;
       }
     }
     s->stream = main_stream;
-    if (rc) (void)hipStreamSynchronize(s->stream2);
+    if (rc && !whole) (void)hipStreamSynchronize(s->stream2);
     s->spikes = saved_spikes;
     s->tune_resident = saved_resident;
     if (rc) return rc;
@@ -1658,17 +1674,32 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
   const int nseg = (K + Kc - 1) / Kc;
   int rc;
-  if ((rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)(nseg * ckpt_doubles(s))))) return rc;
+  // the whole tape (segment seg at seg * tape_len, the layout the backward's
+  // recompute writes into one segment buffer), or a checkpoint per segment
+  const long long tape_len = (long long)Kc * (2 * Model::NS + 1) * s->N;
+  const long long whole_len = (long long)nseg * tape_len;
+  bool whole = s->tune_adjoint_tape == FMS_ADJOINT_TAPE_WHOLE;
+  if (s->tune_adjoint_tape == FMS_ADJOINT_TAPE_AUTO) {
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    // the tape held now is freed before a larger one is allocated
+    const double avail = (double)free_b + (double)s->tape_cap * sizeof(double);
+    whole = (double)whole_len * sizeof(double) <= 0.5 * avail;
+  }
+  s->adj_whole = 0;
+  s->adj_K = 0;   // no backward until this forward has been enqueued
+  if (whole && (rc = ensure(&s->d_tape, s->tape_cap, whole_len))) return rc;
+  if (!whole && (rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)(nseg * ckpt_doubles(s))))) return rc;
   if (ensure_flags(s, n0D)) return FMS_ERR_HIP;
   if ((rc = upload_steps(s, steps, K))) return rc;
   s->adj_steps.assign(steps, steps + 4 * (size_t)K);
   s->adj_n0D = n0D;
   s->adj_interval = Kc;
-  s->adj_K = K;
   for (int seg = 0; seg < nseg; seg++) {
     const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
-    if ((rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), true))) return rc;
+    if (!whole && (rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), true))) return rc;
     fms::DevTrace tr{};
+    if (whole) tr.tape = s->d_tape + (size_t)seg * (size_t)tape_len;
     tr.sensor_slot = s->d_sensor_slot;
     tr.trace = d_traces + (size_t)k0 * F * S;
     tr.n_sensors = S;
@@ -1678,6 +1709,8 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
       rc = launch_fused<double, fms::ExpLibm>(s, s->d_steps + 4 * (size_t)k0, kc, n0D, tr);
     if (rc) return rc;
   }
+  s->adj_whole = whole;
+  s->adj_K = K;
   return 0;
 }
 
@@ -1767,6 +1800,15 @@ int fms_adjoint_backward(FmsSolver *s, const double *dtraces, double *grad_param
   if (grad_params) HIP_TRY(hipMemcpy(grad_params, s->d_grad, 3 * (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost));
   if (grad_prefactor)
     HIP_TRY(hipMemcpy(grad_prefactor, s->d_pfbar, (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+  FMS_API_END
+}
+
+int fms_adjoint_whole_tape(FmsSolver *s, int *whole) {
+  FMS_API_BEGIN
+  if (!s || !whole) return fail(FMS_ERR_ARGUMENT, "null argument");
+  if (s->adj_K <= 0) return fail(FMS_ERR_STATE, "call fms_adjoint_forward first");
+  *whole = s->adj_whole;
   return 0;
   FMS_API_END
 }

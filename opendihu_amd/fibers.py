@@ -168,7 +168,7 @@ class FiberBatch:
                    threads: int | None = None, graph_steps: int | None = None,
                    reaction_per_fibre: int | None = None, pair_halves: int | None = None,
                    reaction_threads: int | None = None, adjoint: int | None = None,
-                   adjoint_threads: int | None = None) -> None:
+                   adjoint_threads: int | None = None, adjoint_tape: int | None = None) -> None:
         """Schedule knobs. Fused kernel: split steps per launch; gates in registers (1),
         streamed (0) or auto (-1); threads per fibre workgroup (128 or 256). Split
         kernels: steps per captured hipGraph (0 launches eagerly); for large models
@@ -178,12 +178,15 @@ class FiberBatch:
         Adjoint backward: segment recompute beside the reverse sweep (0), the same
         with the reverse sweep's priority raised (1) or the recompute's (3), or one
         after the other (2); threads per fibre workgroup of the reverse sweep (256,
-        512 or 1024)."""
+        512 or 1024); the adjoint forward records the whole tape (1) or keeps
+        checkpoints that the backward recomputes from (0), or the whole tape when it
+        fits in half of the free device memory (-1, default)."""
         for key, value in ((L.FMS_TUNE_STEPS_PER_LAUNCH, steps_per_launch), (L.FMS_TUNE_RESIDENT_GATES, resident_gates),
                            (L.FMS_TUNE_THREADS, threads), (L.FMS_TUNE_GRAPH_STEPS, graph_steps),
                            (L.FMS_TUNE_REACTION_PER_FIBRE, reaction_per_fibre),
                            (L.FMS_TUNE_PAIR_HALVES, pair_halves), (L.FMS_TUNE_REACTION_THREADS, reaction_threads),
-                           (L.FMS_TUNE_ADJOINT, adjoint), (L.FMS_TUNE_ADJOINT_THREADS, adjoint_threads)):
+                           (L.FMS_TUNE_ADJOINT, adjoint), (L.FMS_TUNE_ADJOINT_THREADS, adjoint_threads),
+                           (L.FMS_TUNE_ADJOINT_TAPE, adjoint_tape)):
             if value is not None:
                 self._chk(self.lib.fms_set_tuning(self.handle, key, int(value)), "fms_set_tuning")
 

@@ -10,7 +10,7 @@ from cases import AM, CM, uneven
 pytestmark = pytest.mark.gpu
 
 
-def _problem(n_fibers=4, n_steps=600, interval=100, scheme="CN", seed=6, lo=200, hi=400, n_sensors=8):
+def _problem(n_fibers=4, n_steps=600, interval=100, scheme="CN", seed=6, lo=200, hi=400, n_sensors=8, tape="auto"):
     from opendihu_amd.adjoint import AdjointProblem
     from opendihu_amd.fibers import FiberBatch
     mesh, stim = uneven(n_fibers, lo=lo, hi=hi, seed=seed)
@@ -24,7 +24,7 @@ def _problem(n_fibers=4, n_steps=600, interval=100, scheme="CN", seed=6, lo=200,
     dt = 1e-3
     steps = np.stack([np.arange(n_steps) * dt, np.full(n_steps, dt), np.full(n_steps, dt / 2), np.full(n_steps, dt)], 1)
     sensors = np.stack([np.round(np.linspace(0, k - 1, n_sensors)).astype(np.int32) for k in n])
-    prob = AdjointProblem(b, steps, sensors, checkpoint_interval=interval)
+    prob = AdjointProblem(b, steps, sensors, checkpoint_interval=interval, tape=tape)
     rng = np.random.default_rng(seed + 1)
     theta_true = np.stack([rng.uniform(100, 140, n_fibers), rng.uniform(30, 42, n_fibers),
                            rng.uniform(0.2, 0.4, n_fibers), rng.uniform(3.0, 4.5, n_fibers)], 1)
@@ -122,12 +122,14 @@ def test_torch_autograd_function():
     np.testing.assert_allclose(th.grad.numpy(), expect, rtol=1e-12, atol=0)
 
 
-def test_checkpoint_interval_does_not_change_the_gradient():
+@pytest.mark.parametrize("tape", ["whole", "segments"])
+def test_checkpoint_interval_does_not_change_the_gradient(tape):
     """bit-identical gradients for one 1300-step segment and for segments of
     37 and 650 steps"""
-    prob, theta_true, theta0 = _problem(n_steps=1300, interval=1300)
+    prob, theta_true, theta0 = _problem(n_steps=1300, interval=1300, tape=tape)
     target = prob.forward(theta_true)
     tr = prob.forward(theta0)
+    assert prob.whole_tape() == (tape == "whole")
     g1 = prob.backward(tr - target)
     for interval in (37, 650):
         prob.interval = interval
@@ -210,11 +212,35 @@ def test_two_outputs_in_one_loss():
     np.testing.assert_allclose(tb.grad.cpu().numpy(), gb, rtol=1e-12, atol=0)
 
 
+def test_whole_tape_equals_checkpoints():
+    """the forward recording the whole tape (the backward only reverses it)
+    and the forward keeping checkpoints (the backward recomputes every segment,
+    the last one partial): the same traces and bit-identical gradients; at this
+    size the automatic choice is the whole tape"""
+    prob, theta_true, theta0 = _problem(n_steps=500, interval=64, tape="auto")
+    target = prob.forward(theta_true)
+    assert prob.whole_tape()
+    out = {}
+    for tape in ("whole", "segments", "whole"):
+        prob.set_tape(tape)
+        tr = prob.forward(theta0)
+        assert prob.whole_tape() == (tape == "whole")
+        g = prob.backward(tr - target)
+        if tape in out:
+            np.testing.assert_array_equal(g, out[tape][1])
+        out[tape] = (tr, g)
+    np.testing.assert_array_equal(out["whole"][0], out["segments"][0])
+    np.testing.assert_array_equal(out["whole"][1], out["segments"][1])
+    assert np.all(np.isfinite(out["whole"][1])) and np.any(out["whole"][1] != 0)
+    choice = prob.tune_backward(theta0, out["whole"][0] - target)
+    assert choice["tape"] == "whole" and choice["schedule"] == "whole_tape" and len(choice["seconds"]) == 2
+
+
 def test_adjoint_schedules_give_the_same_gradient():
     """the segment recompute beside the reverse sweep (with or without raised
     priority) or before it: bit-identical gradients"""
     from opendihu_amd import _lib as L
-    prob, theta_true, theta0 = _problem(n_steps=500, interval=64)
+    prob, theta_true, theta0 = _problem(n_steps=500, interval=64, tape="segments")
     target = prob.forward(theta_true)
     tr = prob.forward(theta0)
     grads = []
@@ -226,7 +252,7 @@ def test_adjoint_schedules_give_the_same_gradient():
         np.testing.assert_array_equal(g, grads[0])
     choice = prob.tune_backward(theta0, tr - target)
     assert choice["schedule"] in ("overlap", "overlap_priority", "serial", "overlap_recompute_priority")
-    assert choice["threads"] in (256, 512) and len(choice["seconds"]) == 8
+    assert choice["threads"] in (256, 512) and len(choice["seconds"]) == 8 and choice["tape"] == "segments"
     # the reverse sweep with 256 or 512 threads per fibre: the partitioned solve
     # takes other chunks and the prefactor gradient is summed over another
     # number of waves, so the gradients agree to rounding
