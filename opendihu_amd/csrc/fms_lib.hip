@@ -1557,10 +1557,14 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     // the whole forward's tape (segment seg at seg * tape_len), or two segment
     // buffers: the recompute of segment seg-1 (stream2) runs beside the reverse
     // of segment seg (stream)
-    const size_t tape_len = (size_t)Kc * (2 * Model::NS + 1) * N;
-    const int nseg = (K + Kc - 1) / Kc;
     const bool whole = s->adj_whole != 0;
-    if (whole && s->tape_cap < (long long)nseg * (long long)tape_len)
+    // segments of Kc steps (checkpoints), or of the solver's steps per fused
+    // launch (whole tape: step k's entries at k * step_len)
+    const int L = whole ? s->tune_chunk : Kc;
+    const size_t step_len = (size_t)(2 * Model::NS + 1) * N;
+    const size_t tape_len = (size_t)Kc * step_len;
+    const int nseg = (int)((K + (long long)L - 1) / L);
+    if (whole && s->tape_cap < (long long)K * (long long)step_len)
       return fail(FMS_ERR_STATE, "the adjoint forward's tape is gone; run fms_adjoint_forward again");
     if (!whole && (rc = ensure(&s->d_tape, s->tape_cap, 2 * (long long)tape_len))) return rc;
     if ((rc = upload_steps(s, s->adj_steps.data(), K))) return rc;
@@ -1585,9 +1589,9 @@ template <class E> int adjoint_backward_core(FmsSolver *s, const double *d_dtrac
     const size_t lds = (size_t)(fms::pcr_lds_doubles<1024>() + 3 * s->nmax) * sizeof(double);
     hipStream_t main_stream = s->stream;
     for (int idx = 0, seg = nseg - 1; seg >= 0; seg--, idx++) {
-      const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
+      const int k0 = seg * L, kc = std::min(L, K - k0);
       const int buf = idx & 1;
-      double *tape = s->d_tape + (whole ? (size_t)seg : (size_t)buf) * tape_len;
+      double *tape = s->d_tape + (whole ? (size_t)k0 * step_len : (size_t)buf * tape_len);
       if (!whole) {
         // recompute the segment from its checkpoint on stream2, recording the tape;
         // the buffer was last read by the reverse of segment seg+2 (serialised
@@ -1674,10 +1678,11 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   const int F = s->F, S = s->n_sensors, K = n_steps, Kc = checkpoint_interval;
   const int nseg = (K + Kc - 1) / Kc;
   int rc;
-  // the whole tape (segment seg at seg * tape_len, the layout the backward's
-  // recompute writes into one segment buffer), or a checkpoint per segment
-  const long long tape_len = (long long)Kc * (2 * Model::NS + 1) * s->N;
-  const long long whole_len = (long long)nseg * tape_len;
+  // the whole tape (step k's entries at k * (2 NS + 1) * N, the layout the
+  // backward's recompute writes into a segment buffer), or a checkpoint per
+  // segment of Kc steps
+  const long long step_len = (long long)(2 * Model::NS + 1) * s->N;
+  const long long whole_len = (long long)K * step_len;
   bool whole = s->tune_adjoint_tape == FMS_ADJOINT_TAPE_WHOLE;
   if (s->tune_adjoint_tape == FMS_ADJOINT_TAPE_AUTO) {
     size_t free_b = 0, total_b = 0;
@@ -1695,11 +1700,14 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   s->adj_steps.assign(steps, steps + 4 * (size_t)K);
   s->adj_n0D = n0D;
   s->adj_interval = Kc;
-  for (int seg = 0; seg < nseg; seg++) {
-    const int k0 = seg * Kc, kc = std::min(Kc, K - k0);
+  // with the whole tape nothing ties a launch to a segment: the solver's
+  // steps per fused launch
+  const int len = whole ? s->tune_chunk : Kc;
+  for (int seg = 0; seg * (long long)len < K; seg++) {
+    const int k0 = seg * len, kc = std::min(len, K - k0);
     if (!whole && (rc = copy_state(s, s->d_ckpt + (size_t)seg * ckpt_doubles(s), true))) return rc;
     fms::DevTrace tr{};
-    if (whole) tr.tape = s->d_tape + (size_t)seg * (size_t)tape_len;
+    if (whole) tr.tape = s->d_tape + (size_t)k0 * (size_t)step_len;
     tr.sensor_slot = s->d_sensor_slot;
     tr.trace = d_traces + (size_t)k0 * F * S;
     tr.n_sensors = S;

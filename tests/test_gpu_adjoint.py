@@ -221,14 +221,20 @@ def test_whole_tape_equals_checkpoints():
     target = prob.forward(theta_true)
     assert prob.whole_tape()
     out = {}
-    for tape in ("whole", "segments", "whole"):
+    # with the whole tape the forward and the reverse sweep run in launches of
+    # the solver's steps per fused launch (512: one launch; 48: ten and a
+    # partial one), with checkpoints in the 64-step segments
+    for tape, per_launch in (("whole", 512), ("segments", 512), ("whole", 48)):
         prob.set_tape(tape)
+        prob.batch.set_tuning(steps_per_launch=per_launch)
         tr = prob.forward(theta0)
         assert prob.whole_tape() == (tape == "whole")
         g = prob.backward(tr - target)
         if tape in out:
+            np.testing.assert_array_equal(tr, out[tape][0])
             np.testing.assert_array_equal(g, out[tape][1])
         out[tape] = (tr, g)
+    prob.batch.set_tuning(steps_per_launch=512)
     np.testing.assert_array_equal(out["whole"][0], out["segments"][0])
     np.testing.assert_array_equal(out["whole"][1], out["segments"][1])
     assert np.all(np.isfinite(out["whole"][1])) and np.any(out["whole"][1] != 0)
