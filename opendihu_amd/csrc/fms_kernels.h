@@ -583,6 +583,15 @@ __global__ __launch_bounds__(64) void k_thomas_serial(DevFibers fb, double *vm, 
 // GIVEN_RHS: lv holds the right-hand side itself (adjoint solves) instead of the
 // Vm from which the right-hand side B*Vm is assembled.
 // ----------------------------------------------------------------------------
+// 1/b by the hardware reciprocal and two Newton steps (within about one ulp,
+// not correctly rounded): the partitioned solve is held to a tolerance against
+// the serial recurrence, not to its bits, and multiplies by reciprocals
+__device__ __forceinline__ double recip_nr(double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double r1 = __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+  return __builtin_fma(r1, __builtin_fma(-b, r1, 1.0), r1);
+}
+
 template <int T, int MMAX, bool GIVEN_RHS = false>
 __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double *h, double prefactor, double dt,
                                                 bool ie, double *red, int tid) {
@@ -604,11 +613,12 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
         assemble_row(lo + k, n, lv, h, prefactor, dt, ie, a, b, c, d);
         if (GIVEN_RHS) d = lv[lo + k];
         if (k < 2) {
-          aa[k] = div_normal(a, b);
-          cc[k] = div_normal(c, b);
-          dd[k] = div_normal(d, b);
+          const double rb = recip_nr(b);   // the first two rows share one reciprocal
+          aa[k] = a * rb;
+          cc[k] = c * rb;
+          dd[k] = d * rb;
         } else {
-          const double r = div_normal(1.0, b - a * cc[k - 1]);
+          const double r = recip_nr(b - a * cc[k - 1]);
           dd[k] = r * (d - a * dd[k - 1]);
           aa[k] = -r * a * aa[k - 1];
           cc[k] = r * c;
@@ -624,7 +634,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
       }
     }
     if (cnt >= 3) {
-      const double r = div_normal(1.0, 1.0 - cc[0] * aa[1]);
+      const double r = recip_nr(1.0 - cc[0] * aa[1]);
       dd[0] = r * (dd[0] - cc[0] * dd[1]);
       aa[0] = r * aa[0];
       cc[0] = -r * cc[0] * cc[1];
@@ -654,7 +664,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     const double al = -aE, ga = -cE;
     A = al * pa;
     C = ga * cO;
-    const double r = div_normal(1.0, 1.0 + al * pc + ga * aO);
+    const double r = recip_nr(1.0 + al * pc + ga * aO);
     D = r * (dE + al * pd + ga * dO);
     V = r * (vE + al * pv + ga * vO);
     U = r * (uE + al * pu + ga * uO);
@@ -668,12 +678,14 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     const int im = (lane - s) & 63, ip = (lane + s) & 63;
     const double Am = __shfl(A, im), Cm = __shfl(C, im), Dm = __shfl(D, im), Vm = __shfl(V, im), Um = __shfl(U, im);
     const double Ap = __shfl(A, ip), Cp = __shfl(C, ip), Dp = __shfl(D, ip), Vp = __shfl(V, ip), Up = __shfl(U, ip);
+    // a row without a neighbour at distance s has al (ga) = 0, which zeroes the
+    // wrapped-around lane's (finite) coefficients: no select per operand
     const double al = hm ? -A : 0.0, ga = hp ? -C : 0.0;
-    const double r = div_normal(1.0, 1.0 + al * (hm ? Cm : 0.0) + ga * (hp ? Ap : 0.0));
-    const double nA = al * (hm ? Am : 0.0), nC = ga * (hp ? Cp : 0.0);
-    const double nD = D + al * (hm ? Dm : 0.0) + ga * (hp ? Dp : 0.0);
-    const double nV = V + al * (hm ? Vm : 0.0) + ga * (hp ? Vp : 0.0);
-    const double nU = U + al * (hm ? Um : 0.0) + ga * (hp ? Up : 0.0);
+    const double r = recip_nr(1.0 + al * Cm + ga * Ap);
+    const double nA = al * Am, nC = ga * Cp;
+    const double nD = D + al * Dm + ga * Dp;
+    const double nV = V + al * Vm + ga * Vp;
+    const double nU = U + al * Um + ga * Up;
     A = nA * r; C = nC * r; D = nD * r; V = nV * r; U = nU * r;
   }
   // x_{2l} = D - V xL - U xR. Block interface: first unknown (lane 0's even
@@ -696,7 +708,7 @@ __device__ __forceinline__ void lds_thomas_part(double *lv, int n, const double 
     for (int w = 0; w < W; w++) {
       const double df = red[6 * w], vf = red[6 * w + 1], uf = red[6 * w + 2];
       const double dl = red[6 * w + 3], vl = red[6 * w + 4], ul = red[6 * w + 5];
-      const double r = div_normal(1.0, 1.0 + vf * be_prev);
+      const double r = recip_nr(1.0 + vf * be_prev);
       gam[w] = r * (df - vf * al_prev);
       del[w] = -r * uf;
       alp[w] = dl - vl * (al_prev + be_prev * gam[w]);
