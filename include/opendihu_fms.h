@@ -236,7 +236,8 @@ int fms_set_point_buffer_order(FmsSolver *solver, int width, const int *nmj /* [
    tape (72 B per node and step, 92 GB for 256 x 1000 nodes over 5000 steps) and
    the backward only reverses it. FMS_ADJOINT_TAPE_AUTO (default): the whole
    tape when it fits in half of the device memory free at the forward, else
-   segments. The mode of the last forward decides the backward's. Same
+   segments. The mode of the last forward decides the backward's. A whole tape
+   stays allocated until fms_destroy or the next forward with checkpoints. Same
    arithmetic, same gradients. */
 #define FMS_TUNE_ADJOINT_TAPE 12
 #define FMS_ADJOINT_TAPE_AUTO (-1)

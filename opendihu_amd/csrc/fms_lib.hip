@@ -1695,6 +1695,13 @@ int adjoint_forward(FmsSolver *s, const double *steps, int n_steps, int n0D, int
   s->adj_K = 0;   // no backward until this forward has been enqueued
   if (whole && (rc = ensure(&s->d_tape, s->tape_cap, whole_len))) return rc;
   if (!whole && (rc = ensure(&s->d_ckpt, s->ckpt_cap, (long long)(nseg * ckpt_doubles(s))))) return rc;
+  // a whole tape of an earlier forward is released when checkpoints take over
+  // (the backward then needs two segment buffers only)
+  if (!whole && s->tape_cap > 2 * (long long)Kc * step_len) {
+    HIP_TRY(hipFree(s->d_tape));
+    s->d_tape = nullptr;
+    s->tape_cap = 0;
+  }
   if (ensure_flags(s, n0D)) return FMS_ERR_HIP;
   if ((rc = upload_steps(s, steps, K))) return rc;
   s->adj_steps.assign(steps, steps + 4 * (size_t)K);
