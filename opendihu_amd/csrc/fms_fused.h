@@ -78,19 +78,13 @@ __device__ __forceinline__ void fused_nodes_half(double *V, G (*g)[M::NS - 1], c
 // One reaction half of one node with its gates streamed from HBM: load the
 // gates, n0D Heun steps, store the gates (if `store`), return Vm. The caller
 // runs it on every lane of the wave (lanes past the fibre's end pass
-// store = false). Not inlined: inlined into the fused kernel (beside its LDS
-// solve and barriers) the 56-state model's reaction computed wrong, in some
-// builds run-to-run different, states on gfx950 with hipcc of ROCm 7.2 under
-// both the default and the ILP scheduler, while the same code as a called
-// function, or in a kernel of its own (k_reaction, k_reaction_fibre), gives
-// the oracle's bits. The call costs scratch spills (the callee has 256 VGPRs
-// and no AGPRs), so for large models the split kernels are the faster exact
-// schedule and the autotuner picks them.
+// store = false).
 template <class M, class G, class E>
-__device__ __noinline__ double streamed_half(G *__restrict__ gates, long long gstride, long long node, bool store, double V,
-                                             const double *__restrict__ fparams, int stim_mode, int stim_param,
-                                             double amplitude, bool inset, const unsigned char *flags, double t0,
-                                             double dt, int n0D, double vstim) {
+__device__ __forceinline__ double streamed_half_body(G *__restrict__ gates, long long gstride, long long node, bool store,
+                                                     double V, const double *__restrict__ fparams, int stim_mode,
+                                                     int stim_param, double amplitude, bool inset,
+                                                     const unsigned char *flags, double t0, double dt, int n0D,
+                                                     double vstim) {
   G g[M::NS - 1];
 #pragma unroll
   for (int k = 0; k < M::NS - 1; k++) g[k] = gates[(long long)k * gstride + node];
@@ -111,6 +105,40 @@ __device__ __noinline__ double streamed_half(G *__restrict__ gates, long long gs
     for (int k = 0; k < M::NS - 1; k++) gates[(long long)k * gstride + node] = g[k];
   }
   return V;
+}
+
+// Large models run the half as a called function: inlined into the fused
+// kernel (beside its LDS solve and barriers) the 56-state model's reaction
+// computed wrong, in some builds run-to-run different, states on gfx950 with
+// hipcc of ROCm 7.2 under both the default and the ILP scheduler, while the
+// same code as a called function, or in a kernel of its own (k_reaction,
+// k_reaction_fibre), gives the oracle's bits. The call costs scratch spills
+// (the callee has 256 VGPRs and no AGPRs), so for large models the split
+// kernels are the faster exact schedule and the autotuner picks them. Small
+// models (at most 8 states) inline it: the same bits as the call, 3.6 % faster
+// (Hodgkin-Huxley, 4096 fibres of 400-2000 nodes).
+template <class M, class G, class E>
+__device__ __noinline__ double streamed_half_call(G *__restrict__ gates, long long gstride, long long node, bool store,
+                                                  double V, const double *__restrict__ fparams, int stim_mode,
+                                                  int stim_param, double amplitude, bool inset,
+                                                  const unsigned char *flags, double t0, double dt, int n0D,
+                                                  double vstim) {
+  return streamed_half_body<M, G, E>(gates, gstride, node, store, V, fparams, stim_mode, stim_param, amplitude, inset,
+                                     flags, t0, dt, n0D, vstim);
+}
+
+template <class M, class G, class E>
+__device__ __forceinline__ double streamed_half(G *__restrict__ gates, long long gstride, long long node, bool store,
+                                                double V, const double *__restrict__ fparams, int stim_mode,
+                                                int stim_param, double amplitude, bool inset,
+                                                const unsigned char *flags, double t0, double dt, int n0D,
+                                                double vstim) {
+  if constexpr (M::NS <= 8)
+    return streamed_half_body<M, G, E>(gates, gstride, node, store, V, fparams, stim_mode, stim_param, amplitude,
+                                       inset, flags, t0, dt, n0D, vstim);
+  else
+    return streamed_half_call<M, G, E>(gates, gstride, node, store, V, fparams, stim_mode, stim_param, amplitude,
+                                       inset, flags, t0, dt, n0D, vstim);
 }
 
 // NPT > 0: gates resident in registers, NPT nodes per thread; NPT == 0: streamed
@@ -169,6 +197,16 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
         scount[q] = spk.count[off + i];
       }
     }
+  }
+  // streamed gates with spike detection: bit b of `below` says whether the
+  // thread's node of pass b (node b*T + tid) was below the threshold at the
+  // last check; the count and first step are read and written only at a
+  // crossing, the previous Vm once at the end of the launch
+  unsigned below = 0;
+  if constexpr (NPT == 0) {
+    if (spk.enabled)
+      for (int base = 0, b = 0; base + tid < n; base += T, b++)
+        below |= (spk.vm_prev[off + base + tid] < spk.threshold ? 1u : 0u) << b;
   }
 
   // flags of the first half step of step 0
@@ -287,12 +325,12 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
           if (slot >= 0) tr.trace[((long long)s * fb.n_fibers + f) * tr.n_sensors + slot] = V;
         }
         if (spk.enabled && valid) {
-          const double vp = spk.vm_prev[off + i];
-          if (vp < spk.threshold && V >= spk.threshold) {
+          const unsigned bit = 1u << (base / T);
+          if ((below & bit) && V >= spk.threshold) {
             spk.count[off + i]++;
             if (spk.first_step[off + i] < 0) spk.first_step[off + i] = spk.step0 + s + 1;
           }
-          spk.vm_prev[off + i] = V;
+          below = V < spk.threshold ? (below | bit) : (below & ~bit);
         }
         if (valid) lv[i] = V;
       }
@@ -312,7 +350,11 @@ __global__ __launch_bounds__(T) void k_fused(DevFibers fb, DevStim st, double *_
     __syncthreads();
   }
 
-  for (int i = tid; i < n; i += T) vm[off + i] = lv[i];
+  for (int i = tid; i < n; i += T) {
+    vm[off + i] = lv[i];
+    // streamed gates: the last check's Vm is the launch's final Vm
+    if (NPT == 0 && spk.enabled) spk.vm_prev[off + i] = lv[i];
+  }
   if constexpr (NPT > 0) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
