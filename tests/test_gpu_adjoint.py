@@ -212,12 +212,13 @@ def test_two_outputs_in_one_loss():
     np.testing.assert_allclose(tb.grad.cpu().numpy(), gb, rtol=1e-12, atol=0)
 
 
-def test_whole_tape_equals_checkpoints():
+@pytest.mark.parametrize("scheme", ["CN", "IE"])
+def test_whole_tape_equals_checkpoints(scheme):
     """the forward recording the whole tape (the backward only reverses it)
     and the forward keeping checkpoints (the backward recomputes every segment,
     the last one partial): the same traces and bit-identical gradients; at this
     size the automatic choice is the whole tape"""
-    prob, theta_true, theta0 = _problem(n_steps=500, interval=64, tape="auto")
+    prob, theta_true, theta0 = _problem(n_steps=500, interval=64, scheme=scheme, tape="auto")
     target = prob.forward(theta_true)
     assert prob.whole_tape()
     out = {}
