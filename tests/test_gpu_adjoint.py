@@ -212,6 +212,22 @@ def test_two_outputs_in_one_loss():
     np.testing.assert_allclose(tb.grad.cpu().numpy(), gb, rtol=1e-12, atol=0)
 
 
+def test_tape_mode_errors_are_reported():
+    """asking for the tape mode before any forward, a backward before any
+    forward and an unknown tape mode are errors, not crashes"""
+    prob, theta_true, theta0 = _problem(n_steps=50, interval=16)
+    with pytest.raises(RuntimeError, match="fms_adjoint_forward first"):
+        prob.whole_tape()
+    with pytest.raises(RuntimeError, match="fms_adjoint_forward first"):
+        prob.backward(np.zeros((50, 4, 8)))
+    with pytest.raises(RuntimeError, match="adjoint tape"):
+        prob.batch.set_tuning(adjoint_tape=2)
+    with pytest.raises(ValueError):
+        prob.set_tape("disk")
+    prob.forward(theta0)
+    assert prob.whole_tape()
+
+
 @pytest.mark.parametrize("scheme", ["CN", "IE"])
 def test_whole_tape_equals_checkpoints(scheme):
     """the forward recording the whole tape (the backward only reverses it)
